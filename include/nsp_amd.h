@@ -1,0 +1,145 @@
+/*
+ * nsp_amd.h -- C ABI of the MI355X (gfx950) local sparse product C = A*B.
+ *
+ * Every entry point takes plain pointers and sizes; device pointers are HIP
+ * device memory on the handle's device, and work is ordered on the handle's
+ * stream.  Each declaration names the reference interface it replaces
+ * (paths relative to the reference tree).
+ *
+ * Conventions
+ *   - CSR, 0-based, rows of B sorted ascending by column and duplicate-free
+ *     (the reference's precondition: nsp/src/nsp.cu:654,1805,2105 and the
+ *     loader's check_and_fix_order, src/distribute.cu:1050-1069).
+ *   - row pointers int32 (NSP_IDX32, the reference's `int *rpt`,
+ *     nsp/inc/nsp.h:57-70) or int64 (NSP_IDX64, for nnz >= 2^31); one width
+ *     for A, B and C in a call.  Columns are always int32.
+ *   - values FP64 (the reference's `real`) or FP32 with FP32 accumulation.
+ *   - C rows come out sorted ascending, structural zeros kept, as the
+ *     reference produces them (nsp/src/nsp.cu:24-90, 1389-1399).
+ *   - Functions return an nsp_status_t; they never exit the process (the
+ *     reference exits through checkCudaErrors, nsp/inc/helper_cuda.h).
+ */
+#ifndef NSP_AMD_H
+#define NSP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  NSP_OK = 0,
+  NSP_ERR_INVALID = 1,      /* bad argument or malformed CSR */
+  NSP_ERR_OOM = 2,          /* device allocation failed */
+  NSP_ERR_OVERFLOW = 3,     /* result does not fit the chosen index width */
+  NSP_ERR_HIP = 4,          /* HIP runtime error */
+  NSP_ERR_STATE = 5,        /* call sequence error (numeric without symbolic) */
+  NSP_ERR_UNSUPPORTED = 6   /* a size this build does not handle */
+} nsp_status_t;
+
+typedef enum { NSP_IDX32 = 0, NSP_IDX64 = 1 } nsp_index_t;
+typedef enum { NSP_F64 = 0, NSP_F32 = 1 } nsp_value_t;
+
+/* A CSR matrix.  row_ptr is int32_t* or int64_t* per nsp_index_t; val is
+ * double* or float* per nsp_value_t.  Mirrors sfCSR's device fields
+ * (d_rpt, d_col, d_val, M, N, nnz; nsp/inc/nsp.h:57-70). */
+typedef struct {
+  int64_t rows;
+  int64_t cols;
+  int64_t nnz;
+  void *row_ptr;
+  int32_t *col;
+  void *val;
+} nsp_csr_t;
+
+typedef struct nsp_handle_s *nsp_handle_t;
+
+/* Statistics of the last product on a handle. */
+typedef struct {
+  int64_t nprod;            /* sum_i sum_{k in A_i} nnz(B_k) */
+  int64_t nnz;              /* nnz(C) */
+  int64_t max_intprod;      /* largest intermediate-product count of a row */
+  int64_t max_row_nnz;      /* largest nnz of a C row */
+  int64_t sym_bin[16];      /* rows per symbolic bin */
+  int64_t num_bin[16];      /* rows per numeric bin */
+  int32_t n_sym_bins, n_num_bins;
+  int32_t sub_lanes;        /* lanes per A entry used by the product loops */
+  int32_t host_syncs;       /* blocking host synchronisations in the call */
+  int32_t sym_launches;     /* symbolic hash kernels launched */
+  int32_t num_launches;     /* numeric hash kernels launched */
+  /* device time of each phase (HIP events on the handle's stream), filled
+   * when the "timing" parameter is 1; milliseconds */
+  float t_bin_ms, t_sym_ms, t_scan_ms, t_num_ms;
+} nsp_stats_t;
+
+/* ---- handle ------------------------------------------------------------ */
+/* Replaces the per-call state of nsp_init_bin/nsp_release_bin
+ * (nsp/src/nsp.cu:248-298: 11 streams and 5 allocations per call): a handle
+ * owns a reusable device workspace and one stream, bound to the device that
+ * is current when it is created. */
+int nsp_create(nsp_handle_t *h);
+int nsp_destroy(nsp_handle_t h);
+int nsp_set_stream(nsp_handle_t h, void *hip_stream);
+const char *nsp_status_string(int status);
+const char *nsp_last_error(nsp_handle_t h);
+int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
+/* Tuning knobs (an autotuner sets these):
+ *   "sub_lanes"    lanes per A entry in the product loops (0 = from avg row
+ *                  length of B, else 1, 4 or 16)
+ *   "sym_factor"   symbolic hash table slots per intermediate product (1, 2)
+ *   "grid_cap"     blocks per CU for persistent launches
+ *   "timing"       1 records HIP events around each phase (see nsp_stats_t) */
+int nsp_set_param(nsp_handle_t h, const char *name, int64_t value);
+
+/* ---- local product ----------------------------------------------------- */
+/* Symbolic phase of C = A*B: row binning by intermediate products, the hash
+ * count of distinct columns per C row, and the exclusive scan into
+ * c_row_ptr (rows+1 entries, caller-allocated device memory).  *c_nnz gets
+ * nnz(C).  Replaces nsp_set_max_bin + nsp_set_row_nnz
+ * (nsp/src/nsp.cu:417-471, 921-1130). */
+int nsp_spgemm_symbolic(nsp_handle_t h, nsp_index_t it, nsp_value_t vt,
+                        const nsp_csr_t *A, const nsp_csr_t *B,
+                        void *c_row_ptr, int64_t *c_nnz);
+
+/* Numeric phase: fills C->col and C->val (caller-allocated, C->nnz entries)
+ * for the A, B and C->row_ptr of the preceding nsp_spgemm_symbolic on this
+ * handle.  Replaces nsp_set_min_bin + nsp_calculate_value_col_bin
+ * (nsp/src/nsp.cu:1241-1300, 2178-2440). */
+int nsp_spgemm_numeric(nsp_handle_t h, nsp_index_t it, nsp_value_t vt,
+                       const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C);
+
+/* Allocation callbacks for the one-shot product; NULL selects hipMalloc /
+ * hipFree. */
+typedef void *(*nsp_alloc_fn)(size_t bytes, void *user);
+typedef void (*nsp_free_fn)(void *ptr, void *user);
+
+/* One-shot product with callee-allocated C (C->row_ptr, C->col, C->val set
+ * on return; host-synchronous like the reference).  Replaces
+ * void nsp_spgemm_kernel_hash(sfCSR *a, sfCSR *b, sfCSR *c)
+ * (nsp/inc/nsp.h:105, nsp/src/nsp.cu:2448-2488). */
+int nsp_spgemm(nsp_handle_t h, nsp_index_t it, nsp_value_t vt,
+               const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C,
+               nsp_alloc_fn alloc, nsp_free_fn dealloc, void *user);
+
+/* Frees a C returned by nsp_spgemm with the default allocator
+ * (CSRm::free, src/CSR.cu:572-621). */
+int nsp_csr_free(nsp_csr_t *C);
+
+/* Intermediate-product count of every row of A*B into intprod (int64,
+ * A->rows entries, device).  Returns the total in *nprod.  Replaces
+ * nsp_set_intprod_num (nsp/src/nsp.cu:306-320). */
+int nsp_intprod(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B,
+                int64_t *intprod, int64_t *nprod);
+
+/* Device-side check of a CSR's preconditions: row_ptr monotone from 0 to nnz,
+ * columns in [0, ncols), rows strictly increasing.  *bad_row gets the first
+ * offending row or -1.  (The reference's CSRm::checkColumnsOrder,
+ * src/CSR.cu:1485-1513.) */
+int nsp_check_csr(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSP_AMD_H */

@@ -1,0 +1,182 @@
+"""nsp_amd -- MI355X (gfx950) local sparse product C = A*B.
+
+Python host side over the C ABI of libnsp_amd.so.  torch supplies device
+memory and the stream; the compute is the library's HIP kernels.
+
+    import nsp_amd
+    A = nsp_amd.DeviceCSR.from_host(host_csr)           # FP64, int32 row_ptr
+    C = nsp_amd.spgemm(A, A)                            # sorted rows, zeros kept
+
+The reference's entry points this mirrors: nsp_spgemm_kernel_hash(sfCSR*,
+sfCSR*, sfCSR*) (nsp/src/nsp.cu:2448) -> ``spgemm``; the per-rank wrapper
+nsparseMGPU_version0/1 (src/spspmpi.cu:266, 371) -> ``nsparse_mgpu_version0``
+and ``nsparse_mgpu_version1_local`` in ``nsp_amd.csr``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import NSP_F32, NSP_F64, NSP_IDX32, NSP_IDX64, NspError
+
+
+def _torch():
+    import torch
+    return torch
+
+
+@dataclass
+class DeviceCSR:
+    """CSR on the GPU: row_ptr int32/int64, col int32, val float64/float32."""
+    row_ptr: "object"
+    col: "object"
+    val: "object"
+    nrows: int
+    ncols: int
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.numel())
+
+    @property
+    def index_type(self) -> int:
+        return NSP_IDX64 if self.row_ptr.dtype == _torch().int64 else NSP_IDX32
+
+    @property
+    def value_type(self) -> int:
+        return NSP_F32 if self.val.dtype == _torch().float32 else NSP_F64
+
+    @classmethod
+    def from_host(cls, h, dtype: str = "f64", index: str = "i32", device: str = "cuda"):
+        torch = _torch()
+        vt = torch.float32 if dtype == "f32" else torch.float64
+        it = torch.int64 if index == "i64" else torch.int32
+        return cls(torch.from_numpy(np.asarray(h.rpt)).to(device=device, dtype=it),
+                   torch.from_numpy(np.asarray(h.col, np.int32)).to(device),
+                   torch.from_numpy(np.asarray(h.val, np.float64)).to(device=device, dtype=vt),
+                   int(h.nrows), int(h.ncols))
+
+    def to_host(self):
+        from .gen import HostCSR
+        return HostCSR(self.row_ptr.cpu().numpy().astype(np.int64), self.col.cpu().numpy(),
+                       self.val.cpu().numpy().astype(np.float64), self.nrows, self.ncols)
+
+    def c_struct(self) -> _lib.nsp_csr_t:
+        return _lib.nsp_csr_t(self.nrows, self.ncols, self.nnz, self.row_ptr.data_ptr(),
+                              self.col.data_ptr() if self.nnz else None,
+                              self.val.data_ptr() if self.nnz else None)
+
+
+class Handle:
+    """Owns an nsp handle (workspace + stream binding).  By default work is
+    ordered on torch's current stream of the current device."""
+
+    def __init__(self, stream=None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("nsp_amd needs a GPU (torch.cuda.is_available() is False)")
+        self.L = _lib.load()
+        self.h = C.c_void_p()
+        self._check(self.L.nsp_create(C.byref(self.h)))
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self.stream = s
+        self._check(self.L.nsp_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def _check(self, status: int):
+        if status != _lib.NSP_OK:
+            msg = self.L.nsp_last_error(self.h).decode() if getattr(self, "h", None) else ""
+            raise NspError(status, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nsp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_param(self, name: str, value: int):
+        self._check(self.L.nsp_set_param(self.h, name.encode(), int(value)))
+
+    def stats(self) -> dict:
+        st = _lib.nsp_stats_t()
+        self._check(self.L.nsp_get_stats(self.h, C.byref(st)))
+        return {"nprod": st.nprod, "nnz": st.nnz, "max_intprod": st.max_intprod,
+                "max_row_nnz": st.max_row_nnz, "sym_bin": list(st.sym_bin[:st.n_sym_bins]),
+                "num_bin": list(st.num_bin[:st.n_num_bins]), "sub_lanes": st.sub_lanes,
+                "host_syncs": st.host_syncs, "sym_launches": st.sym_launches,
+                "num_launches": st.num_launches, "t_bin_ms": st.t_bin_ms,
+                "t_sym_ms": st.t_sym_ms, "t_scan_ms": st.t_scan_ms, "t_num_ms": st.t_num_ms}
+
+    # -- local product ---------------------------------------------------
+    def symbolic(self, A: DeviceCSR, B: DeviceCSR):
+        """Returns (C row_ptr tensor, nnz)."""
+        torch = _torch()
+        _same_types(A, B)
+        rpt = torch.empty(A.nrows + 1, dtype=A.row_ptr.dtype, device=A.row_ptr.device)
+        nnz = C.c_int64()
+        a, b = A.c_struct(), B.c_struct()
+        self._check(self.L.nsp_spgemm_symbolic(self.h, A.index_type, A.value_type, C.byref(a),
+                                               C.byref(b), C.c_void_p(rpt.data_ptr()),
+                                               C.byref(nnz)))
+        return rpt, int(nnz.value)
+
+    def numeric(self, A: DeviceCSR, B: DeviceCSR, rpt, nnz: int, out=None) -> DeviceCSR:
+        torch = _torch()
+        if out is None:
+            col = torch.empty(nnz, dtype=torch.int32, device=A.col.device)
+            val = torch.empty(nnz, dtype=A.val.dtype, device=A.val.device)
+            out = DeviceCSR(rpt, col, val, A.nrows, B.ncols)
+        a, b, c = A.c_struct(), B.c_struct(), out.c_struct()
+        self._check(self.L.nsp_spgemm_numeric(self.h, A.index_type, A.value_type, C.byref(a),
+                                              C.byref(b), C.byref(c)))
+        return out
+
+    def spgemm(self, A: DeviceCSR, B: DeviceCSR) -> DeviceCSR:
+        rpt, nnz = self.symbolic(A, B)
+        return self.numeric(A, B, rpt, nnz)
+
+    def intprod(self, A: DeviceCSR, B: DeviceCSR):
+        torch = _torch()
+        out = torch.empty(max(A.nrows, 1), dtype=torch.int64, device=A.col.device)
+        nprod = C.c_int64()
+        a, b = A.c_struct(), B.c_struct()
+        self._check(self.L.nsp_intprod(self.h, A.index_type, C.byref(a), C.byref(b),
+                                       C.c_void_p(out.data_ptr()), C.byref(nprod)))
+        return out[:A.nrows], int(nprod.value)
+
+    def check_csr(self, X: DeviceCSR) -> int:
+        bad = C.c_int64()
+        x = X.c_struct()
+        self._check(self.L.nsp_check_csr(self.h, X.index_type, C.byref(x), C.byref(bad)))
+        return int(bad.value)
+
+
+def _same_types(A: DeviceCSR, B: DeviceCSR):
+    if A.row_ptr.dtype != B.row_ptr.dtype:
+        raise ValueError("A and B must use the same row-pointer width")
+    if A.val.dtype != B.val.dtype:
+        raise ValueError("A and B must use the same value type")
+
+
+_default = {}
+
+
+def default_handle() -> Handle:
+    torch = _torch()
+    key = (torch.cuda.current_device(), torch.cuda.current_stream().cuda_stream)
+    h = _default.get(key)
+    if h is None:
+        h = _default[key] = Handle()
+    return h
+
+
+def spgemm(A: DeviceCSR, B: DeviceCSR, handle: Handle | None = None) -> DeviceCSR:
+    """C = A*B on the GPU (sorted rows, structural zeros kept)."""
+    return (handle or default_handle()).spgemm(A, B)

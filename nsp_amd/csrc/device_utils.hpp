@@ -1,0 +1,207 @@
+// Device-side building blocks for the gfx950 SpGEMM kernels: wave64 group
+// reductions and scans, LDS open-addressing hash tables, and register
+// bitonic sorts.  Everything here is written for 64-wide wavefronts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nsp {
+
+constexpr int kWave = 64;
+constexpr int32_t kEmpty = -1;           // empty hash slot (columns are >= 0)
+constexpr uint32_t kPadKey = 0xFFFFFFFFu; // sorts after every column
+
+template <int N> struct Log2 { static constexpr int value = 1 + Log2<N / 2>::value; };
+template <> struct Log2<1> { static constexpr int value = 0; };
+
+// Orders LDS accesses of one wavefront.  A wave's LDS instructions execute in
+// issue order, so a compiler barrier is all that is needed between a write by
+// one lane and a read by another lane of the same wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Fibonacci hashing: the top LOG bits of key * 2^32/phi.  Arithmetic
+// progressions of columns (stencil rows: runs at strides 1, m, m^2) spread
+// over the table instead of aliasing as they do under key & (SH-1).
+template <int LOG>
+__device__ __forceinline__ uint32_t hash_slot(int32_t key) {
+  return ((uint32_t)key * 0x9E3779B1u) >> (32 - LOG);
+}
+
+// ---- group (sub-wave) collectives; GS lanes, GS a power of two <= 64 ------
+template <int GS, typename T>
+__device__ __forceinline__ T group_sum(T x) {
+#pragma unroll
+  for (int o = GS / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GS);
+  return x;
+}
+
+template <int GS, typename T>
+__device__ __forceinline__ T group_max(T x) {
+#pragma unroll
+  for (int o = GS / 2; o > 0; o >>= 1) {
+    T y = __shfl_xor(x, o, GS);
+    x = x > y ? x : y;
+  }
+  return x;
+}
+
+// inclusive scan within groups of GS lanes
+template <int GS, typename T>
+__device__ __forceinline__ T group_incl_scan(T x, int glane) {
+#pragma unroll
+  for (int o = 1; o < GS; o <<= 1) {
+    T y = __shfl_up(x, o, GS);
+    if (glane >= o) x += y;
+  }
+  return x;
+}
+
+// ---- LDS hash tables -------------------------------------------------------
+// Symbolic insert: returns 1 if key was new.  Linear probing; SH slots,
+// SH a power of two; keys holds kEmpty in unused slots.  Terminates whenever
+// the number of distinct keys inserted is <= SH.
+template <int LOG>
+__device__ __forceinline__ int hash_insert_key(int32_t* keys, int32_t key) {
+  constexpr uint32_t MASK = (1u << LOG) - 1;
+  uint32_t h = hash_slot<LOG>(key);
+  volatile int32_t* vk = keys;
+  while (true) {
+    int32_t k = vk[h];
+    if (k == key) return 0;
+    if (k == kEmpty) {
+      int32_t old = atomicCAS(&keys[h], kEmpty, key);
+      if (old == kEmpty) return 1;
+      if (old == key) return 0;
+    }
+    h = (h + 1) & MASK;
+  }
+}
+
+// Numeric insert: accumulate v into key's slot (LDS atomic add).
+template <int LOG, typename V>
+__device__ __forceinline__ void hash_accumulate(int32_t* keys, V* vals, int32_t key, V v) {
+  constexpr uint32_t MASK = (1u << LOG) - 1;
+  uint32_t h = hash_slot<LOG>(key);
+  volatile int32_t* vk = keys;
+  while (true) {
+    int32_t k = vk[h];
+    if (k == key) break;
+    if (k == kEmpty) {
+      int32_t old = atomicCAS(&keys[h], kEmpty, key);
+      if (old == kEmpty || old == key) break;
+    }
+    h = (h + 1) & MASK;
+  }
+  __hip_atomic_fetch_add(&vals[h], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Value of a key known to be present.
+template <int LOG, typename V>
+__device__ __forceinline__ V hash_lookup(const int32_t* keys, const V* vals, int32_t key) {
+  constexpr uint32_t MASK = (1u << LOG) - 1;
+  uint32_t h = hash_slot<LOG>(key);
+  while (keys[h] != key) h = (h + 1) & MASK;
+  return vals[h];
+}
+
+// Runtime-sized variants (window kernels pick the table size per launch).
+__device__ __forceinline__ int hash_insert_key_rt(int32_t* keys, int32_t key, int log2sh) {
+  const uint32_t mask = (1u << log2sh) - 1;
+  uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> (32 - log2sh);
+  volatile int32_t* vk = keys;
+  while (true) {
+    int32_t k = vk[h];
+    if (k == key) return 0;
+    if (k == kEmpty) {
+      int32_t old = atomicCAS(&keys[h], kEmpty, key);
+      if (old == kEmpty) return 1;
+      if (old == key) return 0;
+    }
+    h = (h + 1) & mask;
+  }
+}
+
+// ---- register bitonic sort over a group ------------------------------------
+// GS lanes hold E keys each (blocked layout: element g = glane*E + r); sorts
+// the GS*E keys ascending.  GS and E powers of two; all GS lanes active.
+template <int GS, int E>
+__device__ __forceinline__ void group_bitonic_sort(uint32_t (&v)[E], int glane) {
+  constexpr int N = GS * E;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < E) {
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+          if ((r & j) == 0) {
+            const int g = glane * E + r;
+            const bool asc = (g & k) == 0;
+            uint32_t a = v[r], b = v[r | j];
+            bool sw = asc ? (a > b) : (a < b);
+            v[r] = sw ? b : a;
+            v[r | j] = sw ? a : b;
+          }
+        }
+      } else {
+        const int lx = j / E;
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+          const int g = glane * E + r;
+          uint32_t o = (uint32_t)__shfl_xor((int)v[r], lx, GS);
+          const bool lower = (g & j) == 0;
+          const bool asc = (g & k) == 0;
+          uint32_t mn = v[r] < o ? v[r] : o;
+          uint32_t mx = v[r] < o ? o : v[r];
+          v[r] = (lower == asc) ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+// ---- block bitonic sort in LDS -------------------------------------------
+// Sorts n2 (a power of two) uint32 keys in LDS with all BS threads of the block.
+template <int BS>
+__device__ __forceinline__ void block_bitonic_sort_lds(uint32_t* a, int n2) {
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < n2 / 2; t += BS) {
+        // t-th pair: lower index i has bit j clear
+        int i = ((t / j) * 2 * j) + (t % j);
+        int p = i + j;
+        bool asc = (i & k) == 0;
+        uint32_t x = a[i], y = a[p];
+        if (asc ? (x > y) : (x < y)) { a[i] = y; a[p] = x; }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---- block-wide exclusive scan (BS threads, one value each) ---------------
+template <int BS, typename T>
+__device__ __forceinline__ T block_excl_scan(T x, T* smem /* BS/64 + 1 */, T* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T inc = group_incl_scan<64>(x, lane);
+  if (lane == 63) smem[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T s = 0;
+    for (int i = 0; i < BS / 64; i++) { T t = smem[i]; smem[i] = s; s += t; }
+    smem[BS / 64] = s;
+  }
+  __syncthreads();
+  T r = inc - x + smem[w];
+  if (total) *total = smem[BS / 64];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace nsp

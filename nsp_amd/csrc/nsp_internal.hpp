@@ -1,0 +1,95 @@
+// Host-side internals shared by the translation units of libnsp_amd.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/nsp_amd.h"
+
+namespace nsp {
+
+// Row bins.  Symbolic bins are chosen by the intermediate-product count of a
+// row (an upper bound of its nnz), numeric bins by the exact nnz.  Bins
+// 0..5 are handled by sub-wave or wave-per-row kernels with LDS tables,
+// 6..7 by block-per-row kernels, 8 by the column-window kernels.
+constexpr int kBins = 9;
+constexpr int kHeavyBin = 8;
+
+// Upper bound (inclusive) of each symbolic bin, on intermediate products.
+constexpr int64_t kSymBound[kBins] = {32, 64, 128, 256, 512, 1024, 2048, 16384, INT64_MAX};
+// Upper bound (inclusive) of each numeric bin, on row nnz.
+constexpr int64_t kNumBound[kBins] = {16, 32, 64, 128, 256, 512, 2048, 4096, INT64_MAX};
+
+// Per-call counters living in device memory; mirrored to pinned host memory.
+struct Info {
+  int64_t count_sym[16];
+  int64_t cursor_sym[16];
+  int64_t count_num[16];
+  int64_t cursor_num[16];
+  int64_t nprod;
+  int64_t nnz;
+  int64_t max_intprod;
+  int64_t max_nz;
+  uint32_t tile_ticket;
+  uint32_t error;
+  int64_t pad[4];
+};
+
+struct Workspace {
+  void *base = nullptr;
+  size_t bytes = 0;
+};
+
+struct Handle {
+  int device = 0;
+  int num_cu = 256;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string last_error;
+  nsp_stats_t stats{};
+
+  // tuning knobs
+  int sub_lanes = 0;
+  int sym_factor = 1;
+  int grid_cap = 8;
+  bool timing = false;
+  bool timing_pending = false;
+  hipEvent_t events[8] = {};
+
+  // reusable device workspace
+  Workspace ws;
+  Info *d_info = nullptr;
+  Info *h_info = nullptr;  // pinned
+
+  // state carried from symbolic to numeric
+  bool have_plan = false;
+  int64_t plan_rows = -1, plan_nnz = -1;
+  const void *plan_a = nullptr, *plan_b = nullptr;
+  int plan_sub = 16;
+  int64_t num_count[kBins] = {0};
+  int64_t *a_rpt64 = nullptr, *b_rpt64 = nullptr, *c_rpt64 = nullptr, *intprod = nullptr;
+  int32_t *row_nz = nullptr, *perm_sym = nullptr, *perm_num = nullptr;
+  uint8_t *cls = nullptr;
+  uint64_t *tile_status = nullptr;
+  int64_t ntiles = 0;
+};
+
+int set_error(Handle *h, int status, const std::string &msg);
+int collect_timing(Handle *h);
+
+#define NSP_HIP(h, call)                                                              \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return ::nsp::set_error((h), NSP_ERR_HIP,                                        \
+                              std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define NSP_TRY(expr)            \
+  do {                           \
+    int s_ = (expr);             \
+    if (s_ != NSP_OK) return s_; \
+  } while (0)
+
+}  // namespace nsp

@@ -1,0 +1,865 @@
+// Local sparse product C = A*B for gfx950: row binning, symbolic hash count,
+// wave64 decoupled-look-back scan into C's row pointer, numeric hash
+// accumulate with an in-row sort.
+//
+// Reference path replaced (nsp/src/nsp.cu): nsp_set_intprod_num 306-320,
+// nsp_set_bin / nsp_set_row_perm 325-411, symbolic kernels 478-915 and their
+// launcher 921-1130, the CUB exclusive scan 937/1124, numeric binning
+// 1137-1300, numeric kernels 1308-2151 and their launcher 2178-2440, and the
+// driver nsp_spgemm_kernel_hash 2448-2488.
+//
+// Layout of the work: every kernel after binning walks a bin's row list
+// (row ids grouped by bin, order within a bin following row order per
+// wavefront) with a grid-stride loop, so a launch of any grid size covers
+// the bin; the host sizes grids from the bin counts it reads back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "device_utils.hpp"
+#include "nsp_internal.hpp"
+#include "spgemm_internal.hpp"
+
+namespace nsp {
+
+__host__ __device__ inline int sym_bin_of(int64_t ip) {
+  int b = 0;
+  while (b < kBins - 1 && ip > kSymBound[b]) b++;
+  return b;
+}
+__host__ __device__ inline int num_bin_of(int64_t nz) {
+  int b = 0;
+  while (b < kBins - 1 && nz > kNumBound[b]) b++;
+  return b;
+}
+
+constexpr uint8_t kNoBin = 255;
+
+// ---------------------------------------------------------------------------
+// init: zero the per-call counters and the scan's tile status words.
+// (A fill kernel rather than hipMemsetAsync so the sequence stays valid inside
+// a captured graph.)
+__global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t *w = reinterpret_cast<int64_t *>(info);
+  constexpr int nw = sizeof(Info) / sizeof(int64_t);
+  if (t < nw) w[t] = 0;
+  for (int64_t i = t; i < ntiles; i += (int64_t)gridDim.x * blockDim.x) tile_status[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Row info: intermediate products per row (nsp.cu:306-320), symbolic bin,
+// per-bin histogram, nprod, and int64 copies of A's and B's row pointers.
+template <typename RowT>
+__global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
+                                                 const int32_t *__restrict__ acol,
+                                                 const RowT *__restrict__ brpt, int64_t MB,
+                                                 int64_t *__restrict__ arpt64,
+                                                 int64_t *__restrict__ brpt64,
+                                                 int64_t *__restrict__ intprod,
+                                                 int32_t *__restrict__ row_nz,
+                                                 uint8_t *__restrict__ cls, Info *info) {
+  __shared__ unsigned long long hist[kBins];
+  __shared__ unsigned long long s_nprod, s_max;
+  if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
+  if (threadIdx.x == 0) { s_nprod = 0; s_max = 0; }
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  unsigned long long my_sum = 0, my_max = 0;
+  for (int64_t i = t0; i < MA; i += stride) {
+    const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
+    int64_t s = 0;
+    int64_t p = p0;
+    for (; p + 4 <= p1; p += 4) {
+      int32_t k0 = acol[p], k1 = acol[p + 1], k2 = acol[p + 2], k3 = acol[p + 3];
+      s += (int64_t)brpt[k0 + 1] - (int64_t)brpt[k0] + (int64_t)brpt[k1 + 1] - (int64_t)brpt[k1] +
+           (int64_t)brpt[k2 + 1] - (int64_t)brpt[k2] + (int64_t)brpt[k3 + 1] - (int64_t)brpt[k3];
+    }
+    for (; p < p1; p++) {
+      int32_t k = acol[p];
+      s += (int64_t)brpt[k + 1] - (int64_t)brpt[k];
+    }
+    intprod[i] = s;
+    arpt64[i] = p0;
+    if (s == 0) {
+      cls[i] = kNoBin;
+      row_nz[i] = 0;
+    } else {
+      int b = sym_bin_of(s);
+      cls[i] = (uint8_t)b;
+      atomicAdd(&hist[b], 1ull);
+    }
+    my_sum += (unsigned long long)s;
+    my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
+  }
+  for (int64_t j = t0; j <= MB; j += stride) brpt64[j] = (int64_t)brpt[j];
+  if (t0 == 0) arpt64[MA] = (int64_t)arpt[MA];
+  my_sum = group_sum<64>(my_sum);
+  my_max = group_max<64>(my_max);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&s_nprod, my_sum);
+    atomicMax(&s_max, my_max);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBins && hist[threadIdx.x])
+    atomicAdd((unsigned long long *)&info->count_sym[threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x == 0) {
+    atomicAdd((unsigned long long *)&info->nprod, s_nprod);
+    atomicMax((unsigned long long *)&info->max_intprod, s_max);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scatter rows into per-bin lists.  Each wavefront claims one slot range per
+// bin with a single atomic, so rows of a bin stay in row order within a wave.
+__global__ __launch_bounds__(256) void k_scatter(int64_t M, const uint8_t *__restrict__ cls,
+                                                 const int64_t *__restrict__ counts,
+                                                 int64_t *__restrict__ cursor,
+                                                 int32_t *__restrict__ perm) {
+  __shared__ int64_t off[kBins];
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int b = 0; b < kBins; b++) { off[b] = s; s += counts[b]; }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  // all lanes iterate together (uniform trip count per wave) for the ballots
+  for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < M;
+       base += stride) {
+    const int64_t i = base + lane;
+    const int c = (i < M) ? (int)cls[i] : (int)kNoBin;
+#pragma unroll
+    for (int b = 0; b < kBins; b++) {
+      const uint64_t m = __ballot(c == b);
+      if (m == 0) continue;
+      int64_t pos0 = 0;
+      if (lane == __ffsll((unsigned long long)m) - 1)
+        pos0 = (int64_t)atomicAdd((unsigned long long *)&cursor[b], (unsigned long long)__popcll(m));
+      pos0 = __shfl(pos0, __ffsll((unsigned long long)m) - 1);
+      if (c == b) perm[off[b] + pos0 + __popcll(m & lt)] = (int32_t)i;
+    }
+  }
+}
+
+// Rows handled by one launch: bins [lo, hi] of the binned row list, or, in
+// direct mode (direct_rows >= 0, row list NULL), rows [0, direct_rows) in
+// natural order -- the reference's DIRECT mode (nsp/src/nsp.cu:443-452),
+// used when one bin's table fits every row.
+__device__ __forceinline__ void bin_range(const int64_t *counts, int lo, int hi, int64_t direct_rows,
+                                          int64_t &count, int64_t &off) {
+  if (direct_rows >= 0) {
+    count = direct_rows;
+    off = 0;
+    return;
+  }
+  int64_t s = 0;
+  for (int q = 0; q < lo; q++) s += counts[q];
+  off = s;
+  count = 0;
+  for (int q = lo; q <= hi; q++) count += counts[q];
+}
+
+// ---------------------------------------------------------------------------
+// Symbolic, GS lanes per row (sub-wave groups or one wave), SH-slot LDS key
+// table per row, SUB lanes per A entry.  256 threads per block.
+template <int GS, int SUB, int SH>
+__global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                   const Info *__restrict__ info,
+                                                   const int32_t *__restrict__ perm,
+                                                   const int64_t *__restrict__ arpt,
+                                                   const int32_t *__restrict__ acol,
+                                                   const int64_t *__restrict__ brpt,
+                                                   const int32_t *__restrict__ bcol,
+                                                   int32_t *__restrict__ row_nz) {
+  constexpr int RPB = 256 / GS;
+  constexpr int LOG = Log2<SH>::value;
+  constexpr int NSUB = GS / SUB;
+  __shared__ int32_t keys[RPB][SH];
+  int64_t count, off;
+  bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
+  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
+  int32_t *K = keys[slot];
+  for (int64_t base = blockIdx.x * (int64_t)RPB; base < count; base += (int64_t)gridDim.x * RPB) {
+    const int64_t t = base + slot;
+    if (t < count) {
+      const int32_t r = perm ? perm[off + t] : (int32_t)t;
+#pragma unroll
+      for (int s = glane; s < SH; s += GS) K[s] = kEmpty;
+      wave_lds_sync();
+      const int64_t a0 = arpt[r], a1 = arpt[r + 1];
+      int cnt = 0;
+      for (int64_t ja = a0 + glane / SUB; ja < a1; ja += NSUB) {
+        const int32_t k = acol[ja];
+        const int64_t b0 = brpt[k], b1 = brpt[k + 1];
+        for (int64_t jb = b0 + glane % SUB; jb < b1; jb += SUB) cnt += hash_insert_key<LOG>(K, bcol[jb]);
+      }
+      cnt = group_sum<GS>(cnt);
+      if (glane == 0) row_nz[r] = cnt;
+      wave_lds_sync();
+    }
+  }
+}
+
+// Symbolic, one block of BS threads per row.
+template <int BS, int SUB, int SH>
+__global__ __launch_bounds__(BS) void k_sym_block(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                  const Info *__restrict__ info,
+                                                  const int32_t *__restrict__ perm,
+                                                  const int64_t *__restrict__ arpt,
+                                                  const int32_t *__restrict__ acol,
+                                                  const int64_t *__restrict__ brpt,
+                                                  const int32_t *__restrict__ bcol,
+                                                  int32_t *__restrict__ row_nz) {
+  constexpr int LOG = Log2<SH>::value;
+  constexpr int NSUB = BS / SUB;
+  __shared__ int32_t K[SH];
+  __shared__ int red[BS / 64];
+  int64_t count, off;
+  bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
+  for (int64_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const int32_t r = perm ? perm[off + t] : (int32_t)t;
+    for (int s = threadIdx.x; s < SH; s += BS) K[s] = kEmpty;
+    __syncthreads();
+    const int64_t a0 = arpt[r], a1 = arpt[r + 1];
+    int cnt = 0;
+    for (int64_t ja = a0 + threadIdx.x / SUB; ja < a1; ja += NSUB) {
+      const int32_t k = acol[ja];
+      const int64_t b0 = brpt[k], b1 = brpt[k + 1];
+      for (int64_t jb = b0 + threadIdx.x % SUB; jb < b1; jb += SUB) cnt += hash_insert_key<LOG>(K, bcol[jb]);
+    }
+    cnt = group_sum<64>(cnt);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int s = 0;
+      for (int w = 0; w < BS / 64; w++) s += red[w];
+      row_nz[r] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of row_nz into C's row pointer: single pass, tiles claimed
+// in order through a ticket, decoupled look-back by one wavefront over 64
+// predecessor status words at a time.  Status word = (value << 2) | flag,
+// flag 1 = tile aggregate, 2 = inclusive prefix.  Also bins rows by nnz for
+// the numeric pass.
+constexpr int kScanBS = 256, kScanItems = 8, kScanTile = kScanBS * kScanItems;
+
+__device__ __forceinline__ uint64_t ld_status(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename RowT>
+__global__ __launch_bounds__(kScanBS) void k_scan(int64_t M, const int32_t *__restrict__ row_nz,
+                                                  RowT *__restrict__ crpt,
+                                                  int64_t *__restrict__ crpt64,
+                                                  uint8_t *__restrict__ cls,
+                                                  uint64_t *__restrict__ status, Info *info) {
+  __shared__ int64_t s_wsum[kScanBS / 64 + 1];
+  __shared__ int64_t s_prefix;
+  __shared__ uint32_t s_tile;
+  __shared__ unsigned long long hist[kBins];
+  __shared__ unsigned long long s_max;
+  if (threadIdx.x == 0) {
+    s_tile = atomicAdd(&info->tile_ticket, 1u);
+    s_max = 0;
+  }
+  if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t tile = s_tile;
+  const int64_t base = tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int64_t tsum = 0;
+  unsigned long long tmax = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; q++) {
+    const int64_t i = base + q;
+    v[q] = (i < M) ? row_nz[i] : 0;
+    tsum += v[q];
+    if (i < M) {
+      if (v[q] == 0) {
+        cls[i] = kNoBin;
+      } else {
+        const int b = num_bin_of(v[q]);
+        cls[i] = (uint8_t)b;
+        atomicAdd(&hist[b], 1ull);
+      }
+      tmax = tmax > (unsigned long long)v[q] ? tmax : (unsigned long long)v[q];
+    }
+  }
+  int64_t agg;
+  int64_t texcl = block_excl_scan<kScanBS, int64_t>(tsum, s_wsum, &agg);
+  // look-back
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    if (tile == 0) {
+      if (lane == 0) {
+        st_status(&status[0], ((uint64_t)agg << 2) | 2u);
+        s_prefix = 0;
+      }
+    } else {
+      if (lane == 0) st_status(&status[tile], ((uint64_t)agg << 2) | 1u);
+      int64_t excl = 0;
+      int64_t j = tile - 1;
+      while (true) {
+        const int64_t idx = j - lane;
+        uint64_t s = (idx >= 0) ? ld_status(&status[idx]) : 2u;
+        while ((s & 3u) == 0) s = ld_status(&status[idx]);
+        const uint64_t incl = __ballot((s & 3u) == 2u);
+        const int64_t val = (int64_t)(s >> 2);
+        if (incl) {
+          const int first = __ffsll((unsigned long long)incl) - 1;
+          excl += group_sum<64>(lane <= first ? val : (int64_t)0);
+          break;
+        }
+        excl += group_sum<64>(val);
+        j -= 64;
+      }
+      if (lane == 0) {
+        st_status(&status[tile], ((uint64_t)(excl + agg) << 2) | 2u);
+        s_prefix = excl;
+      }
+    }
+  }
+  tmax = group_max<64>(tmax);
+  if ((threadIdx.x & 63) == 0) atomicMax(&s_max, tmax);
+  __syncthreads();
+  // The grid has floor(M / tile) + 1 tiles, so exactly one thread holds
+  // index M (the end of C's row pointer, i.e. nnz(C)).
+  int64_t run = s_prefix + texcl;
+#pragma unroll
+  for (int q = 0; q < kScanItems; q++) {
+    const int64_t i = base + q;
+    if (i <= M) {
+      crpt[i] = (RowT)run;
+      crpt64[i] = run;
+      if (i == M) {
+        info->nnz = run;
+        if (sizeof(RowT) == 4 && run > (int64_t)INT32_MAX) info->error = NSP_ERR_OVERFLOW;
+      }
+    }
+    run += v[q];
+  }
+  if (threadIdx.x < kBins && hist[threadIdx.x])
+    atomicAdd((unsigned long long *)&info->count_num[threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x == 0) atomicMax((unsigned long long *)&info->max_nz, s_max);
+}
+
+// ---------------------------------------------------------------------------
+// Numeric, GS lanes per row, SH-slot LDS table (keys + values) per row, then
+// compaction of the occupied slots, a register bitonic sort of the row's
+// columns (GS lanes x E keys) and a lookup of each sorted column's value.
+template <typename V, int GS, int SUB, int SH>
+__global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                   const Info *__restrict__ info,
+                                                   const int32_t *__restrict__ perm,
+                                                   const int64_t *__restrict__ arpt,
+                                                   const int32_t *__restrict__ acol,
+                                                   const V *__restrict__ aval,
+                                                   const int64_t *__restrict__ brpt,
+                                                   const int32_t *__restrict__ bcol,
+                                                   const V *__restrict__ bval,
+                                                   const int64_t *__restrict__ crpt,
+                                                   int32_t *__restrict__ ccol,
+                                                   V *__restrict__ cval) {
+  constexpr int RPB = 256 / GS;
+  constexpr int LOG = Log2<SH>::value;
+  constexpr int NSUB = GS / SUB;
+  constexpr int NS = SH / 2;      // sort width
+  constexpr int E = NS / GS;      // keys per lane in the sort
+  constexpr int SPL = SH / GS;    // table slots per lane in the compaction
+  static_assert(E >= 1, "sort width below group size");
+  __shared__ int32_t keys[RPB][SH];
+  __shared__ V vals[RPB][SH];
+  __shared__ uint32_t stage[RPB][NS];
+  int64_t count, off;
+  bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
+  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
+  int32_t *K = keys[slot];
+  V *W = vals[slot];
+  uint32_t *S = stage[slot];
+  for (int64_t base = blockIdx.x * (int64_t)RPB; base < count; base += (int64_t)gridDim.x * RPB) {
+    const int64_t t = base + slot;
+    if (t < count) {
+      const int32_t r = perm ? perm[off + t] : (int32_t)t;
+#pragma unroll
+      for (int s = glane; s < SH; s += GS) {
+        K[s] = kEmpty;
+        W[s] = (V)0;
+      }
+      wave_lds_sync();
+      const int64_t a0 = arpt[r], a1 = arpt[r + 1];
+      for (int64_t ja = a0 + glane / SUB; ja < a1; ja += NSUB) {
+        const int32_t k = acol[ja];
+        const V av = aval[ja];
+        const int64_t b0 = brpt[k], b1 = brpt[k + 1];
+        for (int64_t jb = b0 + glane % SUB; jb < b1; jb += SUB)
+          hash_accumulate<LOG, V>(K, W, bcol[jb], av * bval[jb]);
+      }
+      wave_lds_sync();
+      // compact occupied slots (strided: slot q*GS + glane) into S
+      int c = 0;
+#pragma unroll
+      for (int q = 0; q < SPL; q++) c += (K[q * GS + glane] != kEmpty);
+      int pos = group_incl_scan<GS>(c, glane) - c;
+#pragma unroll
+      for (int q = 0; q < SPL; q++) {
+        const int32_t k = K[q * GS + glane];
+        if (k != kEmpty) S[pos++] = (uint32_t)k;
+      }
+      wave_lds_sync();
+      const int64_t c0 = crpt[r];
+      const int nz = (int)(crpt[r + 1] - c0);
+      uint32_t v[E];
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int g = glane * E + q;
+        v[q] = (g < nz) ? S[g] : kPadKey;
+      }
+      group_bitonic_sort<GS, E>(v, glane);
+#pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int g = glane * E + q;
+        if (g < nz) {
+          ccol[c0 + g] = (int32_t)v[q];
+          cval[c0 + g] = hash_lookup<LOG, V>(K, W, (int32_t)v[q]);
+        }
+      }
+      wave_lds_sync();
+    }
+  }
+}
+
+// Numeric, one block of BS threads per row; table and sort in LDS.
+template <typename V, int BS, int SUB, int SH>
+__global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                  const Info *__restrict__ info,
+                                                  const int32_t *__restrict__ perm,
+                                                  const int64_t *__restrict__ arpt,
+                                                  const int32_t *__restrict__ acol,
+                                                  const V *__restrict__ aval,
+                                                  const int64_t *__restrict__ brpt,
+                                                  const int32_t *__restrict__ bcol,
+                                                  const V *__restrict__ bval,
+                                                  const int64_t *__restrict__ crpt,
+                                                  int32_t *__restrict__ ccol,
+                                                  V *__restrict__ cval) {
+  constexpr int LOG = Log2<SH>::value;
+  constexpr int NSUB = BS / SUB;
+  constexpr int NS = SH / 2;
+  __shared__ int32_t K[SH];
+  __shared__ V W[SH];
+  __shared__ uint32_t S[NS];
+  __shared__ int scan_tmp[BS / 64 + 1];
+  int64_t count, off;
+  bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
+  for (int64_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const int32_t r = perm ? perm[off + t] : (int32_t)t;
+    for (int s = threadIdx.x; s < SH; s += BS) {
+      K[s] = kEmpty;
+      W[s] = (V)0;
+    }
+    __syncthreads();
+    const int64_t a0 = arpt[r], a1 = arpt[r + 1];
+    for (int64_t ja = a0 + threadIdx.x / SUB; ja < a1; ja += NSUB) {
+      const int32_t k = acol[ja];
+      const V av = aval[ja];
+      const int64_t b0 = brpt[k], b1 = brpt[k + 1];
+      for (int64_t jb = b0 + threadIdx.x % SUB; jb < b1; jb += SUB)
+        hash_accumulate<LOG, V>(K, W, bcol[jb], av * bval[jb]);
+    }
+    __syncthreads();
+    constexpr int SPT = SH / BS;
+    int c = 0;
+    for (int q = 0; q < SPT; q++) c += (K[q * BS + threadIdx.x] != kEmpty);
+    int pos = block_excl_scan<BS, int>(c, scan_tmp, nullptr);
+    for (int q = 0; q < SPT; q++) {
+      const int32_t k = K[q * BS + threadIdx.x];
+      if (k != kEmpty) S[pos++] = (uint32_t)k;
+    }
+    const int64_t c0 = crpt[r];
+    const int nz = (int)(crpt[r + 1] - c0);
+    int n2 = 1;
+    while (n2 < nz) n2 <<= 1;
+    for (int g = nz + threadIdx.x; g < n2; g += BS) S[g] = kPadKey;
+    __syncthreads();
+    block_bitonic_sort_lds<BS>(S, n2);
+    for (int g = threadIdx.x; g < nz; g += BS) {
+      const int32_t key = (int32_t)S[g];
+      ccol[c0 + g] = key;
+      cval[c0 + g] = hash_lookup<LOG, V>(K, W, key);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launch tables
+struct GridCfg {
+  int blocks;
+};
+
+static int grid_for(const Handle *h, int64_t items, int per_block, int blocks_per_cu) {
+  int64_t need = (items + per_block - 1) / per_block;
+  int64_t cap = (int64_t)h->num_cu * blocks_per_cu;
+  if (need < 1) need = 1;
+  return (int)std::min(need, cap);
+}
+
+#define NSP_SUB_SWITCH(sub, ...)                                  \
+  switch (sub) {                                                  \
+    case 1: { constexpr int SUB_ = 1; __VA_ARGS__; } break;       \
+    case 4: { constexpr int SUB_ = 4; __VA_ARGS__; } break;       \
+    default: { constexpr int SUB_ = 16; __VA_ARGS__; } break;     \
+  }
+
+// Launches the kernel of bin `bin` over bins [lo, bin] (rows of lower bins fit
+// its larger table), or over all rows in natural order when direct >= 0.
+static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int sub, int64_t count,
+                           const int64_t *arpt, const int32_t *acol, const int64_t *brpt,
+                           const int32_t *bcol) {
+  hipStream_t st = h->stream;
+  const Info *info = h->d_info;
+  const int32_t *perm = direct >= 0 ? nullptr : h->perm_sym;
+  int32_t *rnz = h->row_nz;
+  const int cap = h->grid_cap;
+#define SYM_GROUP(GS, SH)                                                                        \
+  do {                                                                                           \
+    int g = grid_for(h, count, 256 / GS, cap);                                                   \
+    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_sym_group<GS, (SUB_ > GS ? GS : SUB_), SH>),       \
+                                           dim3(g), dim3(256), 0, st, lo, bin, direct, info,     \
+                                           perm, arpt, acol, brpt, bcol, rnz));                              \
+  } while (0)
+#define SYM_BLOCK(BS, SH)                                                                        \
+  do {                                                                                           \
+    int g = grid_for(h, count, 1, cap);                                                          \
+    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_sym_block<BS, SUB_, SH>), dim3(g), dim3(BS), 0,    \
+                                           st, lo, bin, direct, info, perm, arpt, acol, brpt,   \
+                                           bcol, rnz));                                          \
+  } while (0)
+  switch (bin) {
+    case 0: SYM_GROUP(16, 32); break;
+    case 1: SYM_GROUP(32, 64); break;
+    case 2: SYM_GROUP(64, 128); break;
+    case 3: SYM_GROUP(64, 256); break;
+    case 4: SYM_GROUP(64, 512); break;
+    case 5: SYM_GROUP(64, 1024); break;
+    case 6: SYM_BLOCK(256, 2048); break;
+    case 7: SYM_BLOCK(512, 16384); break;
+    default: break;
+  }
+#undef SYM_GROUP
+#undef SYM_BLOCK
+}
+
+template <typename V>
+static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int sub, int64_t count,
+                           const int64_t *arpt, const int32_t *acol, const V *aval,
+                           const int64_t *brpt, const int32_t *bcol, const V *bval,
+                           int32_t *ccol, V *cval) {
+  hipStream_t st = h->stream;
+  const Info *info = h->d_info;
+  const int32_t *perm = direct >= 0 ? nullptr : h->perm_num;
+  const int64_t *crpt = h->c_rpt64;
+  const int cap = h->grid_cap;
+#define NUM_GROUP(GS, SH)                                                                         \
+  do {                                                                                            \
+    int g = grid_for(h, count, 256 / GS, cap);                                                    \
+    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_num_group<V, GS, (SUB_ > GS ? GS : SUB_), SH>),     \
+                                           dim3(g), dim3(256), 0, st, lo, bin, direct, info,      \
+                                           perm, arpt, acol, aval, brpt, bcol, bval, crpt, ccol,  \
+                                           cval));                                                \
+  } while (0)
+#define NUM_BLOCK(BS, SH)                                                                         \
+  do {                                                                                            \
+    int g = grid_for(h, count, 1, cap);                                                           \
+    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_num_block<V, BS, SUB_, SH>), dim3(g), dim3(BS), 0,  \
+                                           st, lo, bin, direct, info, perm, arpt, acol, aval,     \
+                                           brpt, bcol, bval, crpt, ccol, cval));                  \
+  } while (0)
+  switch (bin) {
+    case 0: NUM_GROUP(16, 32); break;
+    case 1: NUM_GROUP(32, 64); break;
+    case 2: NUM_GROUP(64, 128); break;
+    case 3: NUM_GROUP(64, 256); break;
+    case 4: NUM_GROUP(64, 512); break;
+    case 5: NUM_GROUP(64, 1024); break;
+    case 6: NUM_BLOCK(256, 4096); break;
+    case 7: NUM_BLOCK(512, 8192); break;
+    default: break;
+  }
+#undef NUM_GROUP
+#undef NUM_BLOCK
+}
+
+// ---------------------------------------------------------------------------
+// Launch plan over the bins.  Wave-level bins 0..5: if the highest non-empty
+// one holds at least 15% of their rows, its kernel takes all of them (its
+// table fits every smaller row) -- and when no row needs a block-level bin,
+// it walks all rows in natural order with no row list (the reference's
+// DIRECT rule, nsp/src/nsp.cu:443-452).  Block-level bins launch one kernel
+// each.
+struct Launch {
+  int bin, lo;
+  int64_t direct, count;
+};
+
+static int plan_launches(const int64_t *counts, int64_t M, Launch *out) {
+  int n = 0;
+  int bmax = -1;
+  int64_t group_rows = 0, block_rows = 0;
+  for (int b = 0; b <= 5; b++)
+    if (counts[b] > 0) { bmax = b; group_rows += counts[b]; }
+  for (int b = 6; b < kBins; b++) block_rows += counts[b];
+  if (bmax >= 0) {
+    if (counts[bmax] * 100 >= group_rows * 15) {
+      if (block_rows == 0) out[n++] = Launch{bmax, 0, M, M};
+      else out[n++] = Launch{bmax, 0, -1, group_rows};
+    } else {
+      for (int b = bmax; b >= 0; b--)
+        if (counts[b] > 0) out[n++] = Launch{b, b, -1, counts[b]};
+    }
+  }
+  for (int b = kBins - 1; b >= 6; b--)
+    if (counts[b] > 0) out[n++] = Launch{b, b, -1, counts[b]};
+  return n;
+}
+
+// Optional per-phase device timing with HIP events on the handle's stream:
+// 0 start, 1 after binning, 2 after symbolic, 3 after scan, 4/5 numeric.
+static void timing_mark(Handle *h, int i) {
+  if (!h->timing) return;
+  if (!h->events[i]) (void)hipEventCreate(&h->events[i]);
+  (void)hipEventRecord(h->events[i], h->stream);
+}
+
+int collect_timing(Handle *h) {
+  if (!h->timing || !h->timing_pending) return NSP_OK;
+  NSP_HIP(h, hipEventSynchronize(h->events[5]));
+  float t = 0;
+  if (hipEventElapsedTime(&t, h->events[0], h->events[1]) == hipSuccess) h->stats.t_bin_ms = t;
+  if (hipEventElapsedTime(&t, h->events[1], h->events[2]) == hipSuccess) h->stats.t_sym_ms = t;
+  if (hipEventElapsedTime(&t, h->events[2], h->events[3]) == hipSuccess) h->stats.t_scan_ms = t;
+  if (hipEventElapsedTime(&t, h->events[4], h->events[5]) == hipSuccess) h->stats.t_num_ms = t;
+  h->timing_pending = false;
+  return NSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+static int ensure_workspace(Handle *h, int64_t MA, int64_t MB) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const int64_t ntiles = MA / kScanTile + 2;
+  size_t need = al((MA + 1) * 8) * 3 + al((MB + 1) * 8) + al(MA * 4) * 3 + al(MA + 1) + al(ntiles * 8);
+  if (need > h->ws.bytes) {
+    if (h->ws.base) NSP_HIP(h, hipFree(h->ws.base));
+    h->ws.base = nullptr;
+    h->ws.bytes = 0;
+    size_t bytes = need + need / 4;
+    if (hipMalloc(&h->ws.base, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(h, NSP_ERR_OOM, "workspace allocation failed");
+    }
+    h->ws.bytes = bytes;
+  }
+  char *p = (char *)h->ws.base;
+  auto take = [&](size_t b) { char *q = p; p += al(b); return (void *)q; };
+  h->a_rpt64 = (int64_t *)take((MA + 1) * 8);
+  h->c_rpt64 = (int64_t *)take((MA + 1) * 8);
+  h->intprod = (int64_t *)take((MA + 1) * 8);
+  h->b_rpt64 = (int64_t *)take((MB + 1) * 8);
+  h->row_nz = (int32_t *)take(MA * 4);
+  h->perm_sym = (int32_t *)take(MA * 4);
+  h->perm_num = (int32_t *)take(MA * 4);
+  h->cls = (uint8_t *)take(MA + 1);
+  h->tile_status = (uint64_t *)take(ntiles * 8);
+  h->ntiles = ntiles;
+  return NSP_OK;
+}
+
+static int check_csr_args(Handle *h, const char *name, const nsp_csr_t *X) {
+  if (!X) return set_error(h, NSP_ERR_INVALID, std::string(name) + " is NULL");
+  if (X->rows < 0 || X->cols < 0 || X->nnz < 0)
+    return set_error(h, NSP_ERR_INVALID, std::string(name) + " has a negative size");
+  if (X->rows >= INT32_MAX || X->cols > INT32_MAX)
+    return set_error(h, NSP_ERR_UNSUPPORTED, std::string(name) + " has >= 2^31 rows or columns");
+  if (!X->row_ptr)  // rows+1 entries, so at least one even for an empty matrix
+    return set_error(h, NSP_ERR_INVALID, std::string(name) + ".row_ptr is NULL");
+  if (X->nnz > 0 && (!X->col || !X->val))
+    return set_error(h, NSP_ERR_INVALID, std::string(name) + ".col/.val is NULL");
+  return NSP_OK;
+}
+
+static int pick_sub(const Handle *h, const nsp_csr_t *B) {
+  if (h->sub_lanes == 1 || h->sub_lanes == 4 || h->sub_lanes == 16) return h->sub_lanes;
+  const double avg = B->rows > 0 ? (double)B->nnz / (double)B->rows : 0.0;
+  if (avg < 2.0) return 1;
+  if (avg < 9.0) return 4;
+  return 16;
+}
+
+static int read_info(Handle *h) {
+  NSP_HIP(h, hipMemcpyAsync(h->h_info, h->d_info, sizeof(Info), hipMemcpyDeviceToHost, h->stream));
+  NSP_HIP(h, hipStreamSynchronize(h->stream));
+  h->stats.host_syncs++;
+  return NSP_OK;
+}
+
+// Row info + symbolic binning; leaves h_info filled.  Also used by nsp_intprod.
+template <typename RowT>
+static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B) {
+  const int64_t MA = A->rows, MB = B->rows;
+  NSP_TRY(ensure_workspace(h, MA, MB));
+  timing_mark(h, 0);
+  hipLaunchKernelGGL(k_init, dim3(std::max<int64_t>(1, std::min<int64_t>((h->ntiles + 255) / 256, 1024))),
+                     dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles);
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((std::max(MA, MB + 1) + 255) / 256,
+                                                            (int64_t)h->num_cu * 8));
+  hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr,
+                     A->col, (const RowT *)B->row_ptr, MB, h->a_rpt64, h->b_rpt64, h->intprod,
+                     h->row_nz, h->cls, h->d_info);
+  NSP_HIP(h, hipGetLastError());
+  return NSP_OK;
+}
+
+template <typename RowT>
+static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void *c_row_ptr,
+                         int64_t *c_nnz) {
+  const int64_t MA = A->rows;
+  h->stats = nsp_stats_t{};
+  h->have_plan = false;
+  h->timing_pending = false;
+  NSP_TRY(run_rowinfo<RowT>(h, A, B));
+  if (MA > 0) {
+    const int gs = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 255) / 256, (int64_t)h->num_cu * 8));
+    hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(256), 0, h->stream, MA, h->cls,
+                       h->d_info->count_sym, h->d_info->cursor_sym, h->perm_sym);
+  }
+  NSP_TRY(read_info(h));
+  const Info &hi = *h->h_info;
+  if (hi.count_sym[kHeavyBin] > 0)
+    return set_error(h, NSP_ERR_UNSUPPORTED,
+                     "rows with more than 16384 intermediate products need the column-window path");
+  const int sub = pick_sub(h, B);
+  timing_mark(h, 1);
+  {
+    Launch L[kBins];
+    const int n = plan_launches(hi.count_sym, MA, L);
+    for (int q = 0; q < n; q++)
+      launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, sub, L[q].count, h->a_rpt64, A->col,
+                     h->b_rpt64, B->col);
+    h->stats.sym_launches = n;
+  }
+  timing_mark(h, 2);
+  NSP_HIP(h, hipGetLastError());
+  if (MA > 0) {
+    int64_t ntile_used = MA / kScanTile + 1;
+    hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)ntile_used), dim3(kScanBS), 0, h->stream, MA,
+                       h->row_nz, (RowT *)c_row_ptr, h->c_rpt64, h->cls, h->tile_status, h->d_info);
+    const int gs = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 255) / 256, (int64_t)h->num_cu * 8));
+    hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(256), 0, h->stream, MA, h->cls,
+                       h->d_info->count_num, h->d_info->cursor_num, h->perm_num);
+    timing_mark(h, 3);
+  } else {
+    timing_mark(h, 3);
+    RowT zero = 0;
+    NSP_HIP(h, hipMemcpyAsync(c_row_ptr, &zero, sizeof(RowT), hipMemcpyHostToDevice, h->stream));
+    int64_t z64 = 0;
+    NSP_HIP(h, hipMemcpyAsync(h->c_rpt64, &z64, 8, hipMemcpyHostToDevice, h->stream));
+  }
+  NSP_HIP(h, hipGetLastError());
+  NSP_TRY(read_info(h));
+  if (hi.error) return set_error(h, (int)hi.error, "C has 2^31 or more entries; use NSP_IDX64");
+  h->stats.nprod = hi.nprod;
+  h->stats.nnz = (MA > 0) ? hi.nnz : 0;
+  h->stats.max_intprod = hi.max_intprod;
+  h->stats.max_row_nnz = hi.max_nz;
+  h->stats.n_sym_bins = kBins;
+  h->stats.n_num_bins = kBins;
+  h->stats.sub_lanes = sub;
+  for (int b = 0; b < kBins; b++) {
+    h->stats.sym_bin[b] = hi.count_sym[b];
+    h->stats.num_bin[b] = hi.count_num[b];
+    h->num_count[b] = hi.count_num[b];
+  }
+  if (hi.count_num[kHeavyBin] > 0)
+    return set_error(h, NSP_ERR_UNSUPPORTED, "rows of C with more than 4096 entries need the column-window path");
+  *c_nnz = h->stats.nnz;
+  h->have_plan = true;
+  h->plan_rows = MA;
+  h->plan_nnz = h->stats.nnz;
+  h->plan_a = A->col;
+  h->plan_b = B->col;
+  h->plan_sub = sub;
+  return NSP_OK;
+}
+
+int spgemm_symbolic(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                    void *c_row_ptr, int64_t *c_nnz) {
+  NSP_TRY(check_csr_args(h, "A", A));
+  NSP_TRY(check_csr_args(h, "B", B));
+  if (vt != NSP_F64 && vt != NSP_F32) return set_error(h, NSP_ERR_INVALID, "unknown value type");
+  if (A->cols != B->rows) return set_error(h, NSP_ERR_INVALID, "A.cols != B.rows");
+  if (!c_row_ptr || !c_nnz) return set_error(h, NSP_ERR_INVALID, "c_row_ptr / c_nnz is NULL");
+  if (it == NSP_IDX32) {
+    if (A->nnz > INT32_MAX || B->nnz > INT32_MAX)
+      return set_error(h, NSP_ERR_OVERFLOW, "nnz >= 2^31 needs NSP_IDX64");
+    return symbolic_impl<int32_t>(h, A, B, c_row_ptr, c_nnz);
+  }
+  if (it == NSP_IDX64) return symbolic_impl<int64_t>(h, A, B, c_row_ptr, c_nnz);
+  return set_error(h, NSP_ERR_INVALID, "unknown index type");
+}
+
+template <typename V>
+static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C) {
+  timing_mark(h, 4);
+  Launch L[kBins];
+  const int n = plan_launches(h->num_count, A->rows, L);
+  for (int q = 0; q < n; q++)
+    launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, h->plan_sub, L[q].count, h->a_rpt64, A->col,
+                      (const V *)A->val, h->b_rpt64, B->col, (const V *)B->val, C->col, (V *)C->val);
+  h->stats.num_launches = n;
+  timing_mark(h, 5);
+  h->timing_pending = h->timing;
+  NSP_HIP(h, hipGetLastError());
+  return NSP_OK;
+}
+
+int spgemm_numeric(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                   nsp_csr_t *C) {
+  if (!h->have_plan) return set_error(h, NSP_ERR_STATE, "nsp_spgemm_numeric without a symbolic phase");
+  if (!A || !B || !C) return set_error(h, NSP_ERR_INVALID, "NULL matrix");
+  if (A->rows != h->plan_rows || A->col != h->plan_a || B->col != h->plan_b)
+    return set_error(h, NSP_ERR_STATE, "A/B differ from the symbolic phase");
+  if (C->nnz != h->plan_nnz) return set_error(h, NSP_ERR_INVALID, "C.nnz differs from the symbolic count");
+  if (C->nnz > 0 && (!C->col || !C->val)) return set_error(h, NSP_ERR_INVALID, "C.col/C.val is NULL");
+  (void)it;
+  if (vt == NSP_F64) return numeric_impl<double>(h, A, B, C);
+  if (vt == NSP_F32) return numeric_impl<float>(h, A, B, C);
+  return set_error(h, NSP_ERR_INVALID, "unknown value type");
+}
+
+int spgemm_intprod(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, int64_t *intprod,
+                   int64_t *nprod) {
+  NSP_TRY(check_csr_args(h, "A", A));
+  NSP_TRY(check_csr_args(h, "B", B));
+  if (A->cols != B->rows) return set_error(h, NSP_ERR_INVALID, "A.cols != B.rows");
+  h->have_plan = false;
+  if (it == NSP_IDX32) NSP_TRY(run_rowinfo<int32_t>(h, A, B));
+  else NSP_TRY(run_rowinfo<int64_t>(h, A, B));
+  if (A->rows > 0 && intprod)
+    NSP_HIP(h, hipMemcpyAsync(intprod, h->intprod, A->rows * 8, hipMemcpyDeviceToDevice, h->stream));
+  NSP_TRY(read_info(h));
+  if (nprod) *nprod = h->h_info->nprod;
+  return NSP_OK;
+}
+
+}  // namespace nsp

@@ -1,0 +1,158 @@
+"""GPU parity of the local product C = A*B against the CPU oracle.
+
+Structure (row_ptr, col_idx) must match bit-exactly; values within 1e-12
+(FP64) or 1e-5 (FP32 with FP32 accumulation) of the FP64 oracle, relative to
+sum_k |a_ik b_kj| per entry.  Every call goes through the C ABI of
+libnsp_amd.so.
+"""
+import numpy as np
+import pytest
+
+from nsp_amd import gen
+from tests.helpers import check_against_oracle, to_dtype
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def nsp():
+    import torch
+    assert torch.cuda.is_available(), "GPU test needs a GPU"
+    import nsp_amd
+    return nsp_amd
+
+
+def run(nsp, A, B, dtype="f64", index="i32", handle=None):
+    dA = nsp.DeviceCSR.from_host(A, dtype, index)
+    dB = nsp.DeviceCSR.from_host(B, dtype, index)
+    h = handle or nsp.Handle()
+    C = h.spgemm(dA, dB)
+    return C, h
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_laplacian_256_closed_form(nsp, orc, dtype):
+    """Config 1: 5-point Laplacian on 256^2, C = A*A."""
+    print("case: laplacian 256 A*A", dtype, flush=True)
+    L = gen.laplacian_5pt(256)
+    C, h = run(nsp, L, L, dtype)
+    st = h.stats()
+    assert st["nprod"] == 1_629_192 and st["nnz"] == 846_852
+    comp, _ = check_against_oracle(L, L, C, dtype, orc)
+    if dtype == "f64":
+        assert comp == 0.0   # small integers: exact
+
+
+@pytest.mark.parametrize("dtype,index", [("f64", "i32"), ("f64", "i64"), ("f32", "i32"), ("f32", "i64")])
+def test_stencil27_small(nsp, orc, dtype, index):
+    print("case: stencil27 m=20", dtype, index, flush=True)
+    A = to_dtype(gen.stencil_27pt(20, seed=5), dtype)
+    C, _ = run(nsp, A, A, dtype, index)
+    check_against_oracle(A, A, C, dtype, orc)
+
+
+def mixed_rows(nrows, ncols, lo, hi, seed):
+    """Rows with log-uniform lengths in [lo, hi], columns uniform."""
+    rng = np.random.default_rng(seed)
+    lens = np.exp(rng.uniform(np.log(lo), np.log(hi + 1), nrows)).astype(np.int64)
+    lens = np.clip(lens, 0, ncols)
+    rows = np.repeat(np.arange(nrows), lens)
+    cols = np.concatenate([rng.choice(ncols, int(k), replace=False) for k in lens]) \
+        if lens.sum() else np.zeros(0, np.int64)
+    vals = rng.uniform(-1, 1, rows.size)
+    return gen._from_rows_cols(nrows, ncols, rows, cols, vals)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_all_bins(nsp, orc, dtype):
+    """Row lengths spread so that every symbolic bin up to 16384 products and
+    every numeric bin up to 4096 entries is populated."""
+    print("case: all bins", dtype, flush=True)
+    A = to_dtype(mixed_rows(3000, 2500, 1, 150, seed=1), dtype)
+    B = to_dtype(mixed_rows(2500, 4000, 1, 100, seed=2), dtype)
+    C, h = run(nsp, A, B, dtype)
+    st = h.stats()
+    assert all(c > 0 for c in st["sym_bin"][:8]), st["sym_bin"]
+    assert all(c > 0 for c in st["num_bin"][:8]), st["num_bin"]
+    check_against_oracle(A, B, C, dtype, orc)
+
+
+@pytest.mark.parametrize("sub", [1, 4, 16])
+def test_sub_lane_variants(nsp, orc, sub):
+    print("case: sub lanes", sub, flush=True)
+    A = mixed_rows(800, 700, 1, 60, seed=3)
+    B = mixed_rows(700, 900, 1, 60, seed=4)
+    h = nsp.Handle()
+    h.set_param("sub_lanes", sub)
+    C, _ = run(nsp, A, B, "f64", handle=h)
+    check_against_oracle(A, B, C, "f64", orc)
+
+
+def test_structural_zeros_and_empty_rows(nsp, orc):
+    print("case: zeros and empty rows", flush=True)
+    A = gen.random_csr(400, 300, 0.02, seed=9, integer_values=True, empty_rows=0.3)
+    B = gen.random_csr(300, 350, 0.02, seed=10, integer_values=True, empty_rows=0.3)
+    C, _ = run(nsp, A, B)
+    comp, _ = check_against_oracle(A, B, C, "f64", orc)
+    assert comp == 0.0
+    # cancellation: two products summing to zero stay as an explicit entry
+    A2 = gen.HostCSR(np.array([0, 2]), np.array([0, 1], np.int32), np.array([1.0, 1.0]), 1, 2)
+    B2 = gen.HostCSR(np.array([0, 1, 2]), np.array([3, 3], np.int32), np.array([2.0, -2.0]), 2, 5)
+    C2, _ = run(nsp, A2, B2)
+    assert C2.row_ptr.cpu().tolist() == [0, 1] and C2.col.cpu().tolist() == [3]
+    assert C2.val.cpu().tolist() == [0.0]
+
+
+def test_empty_inputs(nsp):
+    print("case: empty inputs", flush=True)
+    Z = gen.HostCSR(np.zeros(6, np.int64), np.zeros(0, np.int32), np.zeros(0), 5, 4)
+    B = gen.random_csr(4, 7, 0.5, seed=0)
+    C, _ = run(nsp, Z, B)
+    assert C.row_ptr.cpu().tolist() == [0] * 6 and C.nnz == 0
+    E = gen.HostCSR(np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0), 0, 4)
+    C, _ = run(nsp, E, B)
+    assert C.row_ptr.cpu().tolist() == [0] and C.nnz == 0
+
+
+def test_repeated_calls_reuse_handle(nsp, orc):
+    print("case: handle reuse", flush=True)
+    h = nsp.Handle()
+    for seed in range(3):
+        A = mixed_rows(500 + 300 * seed, 400, 1, 40, seed=seed)
+        B = mixed_rows(400, 600, 1, 40, seed=seed + 7)
+        C, _ = run(nsp, A, B, handle=h)
+        check_against_oracle(A, B, C, "f64", orc)
+
+
+def test_intprod_matches_oracle(nsp, orc):
+    print("case: intprod", flush=True)
+    A = mixed_rows(1000, 800, 1, 50, seed=12)
+    B = gen.random_csr(800, 900, 0.03, seed=13, empty_rows=0.2)
+    h = nsp.Handle()
+    ip, total = h.intprod(nsp.DeviceCSR.from_host(A), nsp.DeviceCSR.from_host(B))
+    ref, rtotal = orc.intprod(A, B)
+    np.testing.assert_array_equal(ip.cpu().numpy(), ref)
+    assert total == rtotal
+
+
+def test_invalid_arguments_raise(nsp):
+    print("case: invalid args", flush=True)
+    A = nsp.DeviceCSR.from_host(gen.random_csr(10, 8, 0.3, seed=1))
+    B = nsp.DeviceCSR.from_host(gen.random_csr(9, 8, 0.3, seed=2))
+    h = nsp.Handle()
+    with pytest.raises(nsp.NspError) as e:
+        h.spgemm(A, B)                      # A.cols != B.rows
+    assert e.value.status == 1
+    with pytest.raises(nsp.NspError):
+        h.numeric(A, A, A.row_ptr, 0)       # numeric without a symbolic phase
+
+
+def test_check_csr(nsp):
+    print("case: check_csr", flush=True)
+    X = gen.random_csr(50, 40, 0.2, seed=3)
+    h = nsp.Handle()
+    assert h.check_csr(nsp.DeviceCSR.from_host(X)) == -1
+    bad = gen.HostCSR(X.rpt.copy(), X.col.copy(), X.val, 50, 40)
+    r = int(np.argmax(np.diff(bad.rpt) >= 2))
+    bad.col[bad.rpt[r]], bad.col[bad.rpt[r] + 1] = bad.col[bad.rpt[r] + 1], bad.col[bad.rpt[r]]
+    assert h.check_csr(nsp.DeviceCSR.from_host(bad)) == r
