@@ -139,6 +139,30 @@ int nsp_intprod(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *A, const nsp_cs
  * src/CSR.cu:1485-1513.) */
 int nsp_check_csr(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row);
 
+/* ---- one rank's local share on one device ------------------------------- */
+/* Column compaction ("shrink"): perm = sorted unique(col U [0..l]),
+ * new_col[e] = position of col[e] in perm, *shrinked_m = |perm|.  Every
+ * col[e] must lie in [lo, hi) (columns are in the rank frame and may be
+ * negative); perm needs at most max(hi, l+1) - min(lo, 0) entries
+ * (perm_cap).  Replaces get_shrinked_col / shrink_col (src/spspmpi.cu:106-
+ * 162), getmct_4shrink (src/getmct.cu:135-220) and
+ * apply_mask_permut_GPU_noSideEffects (src/spspmpi.cu:58-104). */
+int nsp_shrink_cols(nsp_handle_t h, int64_t nnz, const int32_t *col, int64_t l, int64_t lo, int64_t hi,
+                    int32_t *new_col, int32_t *perm, int64_t perm_cap, int64_t *shrinked_m);
+
+/* Row pointer (nr+1 entries, width it) of the CSR made of B rows
+ * rows[i] + row_offset, i < nr (device list); *out_nnz gets its nnz.  With
+ * nsp_gather_rows_fill this assembles completedP on one device, replacing the
+ * halo-row exchange and _completedP_rows2 (src/spspmpi.cu:37-55, 584-906). */
+int nsp_gather_rows_count(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *B, int64_t nr, const int32_t *rows,
+                          int64_t row_offset, void *out_row_ptr, int64_t *out_nnz);
+
+/* Fills out->col (B columns minus col_shift: the column re-shift of
+ * src/spspmpi.cu:822-828) and out->val for the rows counted above;
+ * out->row_ptr is the row pointer from nsp_gather_rows_count. */
+int nsp_gather_rows_fill(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *B, int64_t nr,
+                         const int32_t *rows, int64_t row_offset, int64_t col_shift, nsp_csr_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,6 +157,41 @@ class Handle:
         self._check(self.L.nsp_check_csr(self.h, X.index_type, C.byref(x), C.byref(bad)))
         return int(bad.value)
 
+    # -- one rank's local share -------------------------------------------
+    def shrink_cols(self, col, l: int, lo: int, hi: int):
+        """(new_col, perm): perm = sorted unique(col U [0..l]) (int32 device
+        tensors); every column must lie in [lo, hi)."""
+        torch = _torch()
+        cap = max(hi, l + 1) - min(lo, 0)
+        new_col = torch.empty_like(col)
+        perm = torch.empty(max(cap, 1), dtype=torch.int32, device=col.device)
+        m = C.c_int64()
+        self._check(self.L.nsp_shrink_cols(self.h, col.numel(), C.c_void_p(col.data_ptr()), int(l),
+                                           int(lo), int(hi), C.c_void_p(new_col.data_ptr()),
+                                           C.c_void_p(perm.data_ptr()), cap, C.byref(m)))
+        return new_col, perm[:m.value]
+
+    def gather_rows(self, B: DeviceCSR, rows, row_offset: int = 0, col_shift: int = 0,
+                    ncols: int | None = None) -> DeviceCSR:
+        """CSR of B rows rows[i] + row_offset, columns minus col_shift."""
+        torch = _torch()
+        nr = rows.numel()
+        rpt = torch.empty(nr + 1, dtype=B.row_ptr.dtype, device=B.col.device)
+        nnz = C.c_int64()
+        b = B.c_struct()
+        self._check(self.L.nsp_gather_rows_count(self.h, B.index_type, C.byref(b), nr,
+                                                 C.c_void_p(rows.data_ptr() if nr else None),
+                                                 int(row_offset), C.c_void_p(rpt.data_ptr()),
+                                                 C.byref(nnz)))
+        out = DeviceCSR(rpt, torch.empty(nnz.value, dtype=torch.int32, device=B.col.device),
+                        torch.empty(nnz.value, dtype=B.val.dtype, device=B.col.device), nr,
+                        B.ncols if ncols is None else ncols)
+        o = out.c_struct()
+        self._check(self.L.nsp_gather_rows_fill(self.h, B.index_type, B.value_type, C.byref(b), nr,
+                                                C.c_void_p(rows.data_ptr() if nr else None),
+                                                int(row_offset), int(col_shift), C.byref(o)))
+        return out
+
 
 def _same_types(A: DeviceCSR, B: DeviceCSR):
     if A.row_ptr.dtype != B.row_ptr.dtype:

@@ -57,6 +57,9 @@ SIGNATURES = [
     ("nsp_csr_free", _I, [_CSRP]),
     ("nsp_intprod", _I, [_H, _I, _CSRP, _CSRP, _VP, _P64]),
     ("nsp_check_csr", _I, [_H, _I, _CSRP, _P64]),
+    ("nsp_shrink_cols", _I, [_H, _I64, _VP, _I64, _I64, _I64, _VP, _VP, _I64, _P64]),
+    ("nsp_gather_rows_count", _I, [_H, _I, _CSRP, _I64, _VP, _I64, _VP, _P64]),
+    ("nsp_gather_rows_fill", _I, [_H, _I, _I, _CSRP, _I64, _VP, _I64, _I64, _CSRP]),
 ]
 
 

@@ -108,6 +108,7 @@ int nsp_destroy(nsp_handle_t hs) {
   Handle *h = &hs->h;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->ws.base) (void)hipFree(h->ws.base);
+  if (h->aux.base) (void)hipFree(h->aux.base);
   for (hipEvent_t e : h->events)
     if (e) (void)hipEventDestroy(e);
   if (h->d_info) (void)hipFree(h->d_info);
@@ -257,6 +258,24 @@ int nsp_intprod(nsp_handle_t hs, nsp_index_t it, const nsp_csr_t *A, const nsp_c
 int nsp_check_csr(nsp_handle_t hs, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row) {
   if (!hs) return NSP_ERR_INVALID;
   return nsp::check_csr(&hs->h, it, X, bad_row);
+}
+
+int nsp_shrink_cols(nsp_handle_t hs, int64_t nnz, const int32_t *col, int64_t l, int64_t lo, int64_t hi,
+                    int32_t *new_col, int32_t *perm, int64_t perm_cap, int64_t *shrinked_m) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::shrink_cols(&hs->h, nnz, col, l, lo, hi, new_col, perm, perm_cap, shrinked_m);
+}
+
+int nsp_gather_rows_count(nsp_handle_t hs, nsp_index_t it, const nsp_csr_t *B, int64_t nr, const int32_t *rows,
+                          int64_t row_offset, void *out_row_ptr, int64_t *out_nnz) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::gather_rows_count(&hs->h, it, B, nr, rows, row_offset, out_row_ptr, out_nnz);
+}
+
+int nsp_gather_rows_fill(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *B, int64_t nr,
+                         const int32_t *rows, int64_t row_offset, int64_t col_shift, nsp_csr_t *out) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::gather_rows_fill(&hs->h, it, vt, B, nr, rows, row_offset, col_shift, out);
 }
 
 }  // extern "C"

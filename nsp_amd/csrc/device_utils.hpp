@@ -14,6 +14,13 @@ constexpr uint32_t kPadKey = 0xFFFFFFFFu; // sorts after every column
 template <int N> struct Log2 { static constexpr int value = 1 + Log2<N / 2>::value; };
 template <> struct Log2<1> { static constexpr int value = 0; };
 
+// Relaxed LDS read of a hash slot that other lanes may CAS concurrently (an
+// atomic load keeps it a ds_read; a volatile access becomes a flat load that
+// waits on every outstanding global load).
+__device__ __forceinline__ int32_t lds_ld(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Orders LDS accesses of one wavefront.  A wave's LDS instructions execute in
 // issue order, so a compiler barrier is all that is needed between a write by
 // one lane and a read by another lane of the same wave.
@@ -25,12 +32,96 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
-// Fibonacci hashing: the top LOG bits of key * 2^32/phi.  Arithmetic
-// progressions of columns (stencil rows: runs at strides 1, m, m^2) spread
-// over the table instead of aliasing as they do under key & (SH-1).
+// Multiplicative (Fibonacci) hashing in 24-bit arithmetic: slot = the top LOG
+// bits of (key * floor(2^24/phi)) mod 2^24, with the full-rate 24-bit multiply
+// (v_mul_u32_u24) instead of the quarter-rate 32-bit one and one bit-field
+// extract.  Arithmetic progressions of columns (stencil rows: runs at strides
+// 1, m, m^2) spread over the table instead of aliasing as under key & (SH-1).
+// Only the low 24 bits of a column enter the hash; columns 2^24 apart collide,
+// which costs probes, never correctness.
 template <int LOG>
 __device__ __forceinline__ uint32_t hash_slot(int32_t key) {
-  return ((uint32_t)key * 0x9E3779B1u) >> (32 - LOG);
+  const uint32_t p = (uint32_t)__umul24((uint32_t)key, 0x9E3779u);  // __umul24 returns int
+  return (p & 0xFFFFFFu) >> (24 - LOG);
+}
+
+constexpr int32_t kDummy = -2;  // key of a lane's private dummy slot
+
+// Continues the linear probe of `key` after slot h (its home slot or a slot
+// already found taken by another key); returns the key's slot and sets
+// inserted = 1 when this call claimed the slot.
+template <int LOG>
+__device__ __forceinline__ uint32_t hash_probe_from(int32_t *keys, int32_t key, uint32_t h, int &inserted) {
+  constexpr uint32_t MASK = (1u << LOG) - 1;
+  // bounded: a full table (impossible when the bin bounds hold) ends the walk
+  for (uint32_t step = 0; step <= MASK; step++) {
+    h = (h + 1) & MASK;
+    const int32_t k = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == key) return h;
+    if (k == kEmpty) {
+      const int32_t old = atomicCAS(&keys[h], kEmpty, key);
+      if (old == kEmpty) {
+        inserted = 1;
+        return h;
+      }
+      if (old == key) return h;
+    }
+  }
+  return h;
+}
+
+// Batched inserts of R keys per lane (one window of products).  The first
+// probe of every key is an unconditional CAS on its home slot, all R issued
+// back to back; only keys that find their home slot taken by another key walk
+// on.  Lanes with ok[u] false use their private dummy slot SH + glane (tables
+// hold SH + GS slots), so no branch is needed on the common path.
+// Returns the number of keys this lane inserted for the first time.
+template <int LOG, int R>
+__device__ __forceinline__ int hash_insert_keys(int32_t *keys, const int32_t (&key)[R], const bool (&ok)[R],
+                                                int glane) {
+  constexpr uint32_t SH = 1u << LOG;
+  int32_t kk[R], old[R];
+  uint32_t h[R];
+#pragma unroll
+  for (int u = 0; u < R; u++) {
+    kk[u] = ok[u] ? key[u] : kDummy;
+    h[u] = ok[u] ? hash_slot<LOG>(key[u]) : SH + glane;
+  }
+#pragma unroll
+  for (int u = 0; u < R; u++) old[u] = atomicCAS(&keys[h[u]], kEmpty, kk[u]);
+  int cnt = 0;
+#pragma unroll
+  for (int u = 0; u < R; u++) {
+    int ins = (old[u] == kEmpty) & ok[u];
+    if (old[u] != kEmpty && old[u] != kk[u]) hash_probe_from<LOG>(keys, kk[u], h[u], ins);
+    cnt += ins;
+  }
+  return cnt;
+}
+
+// Batched accumulate: finds or inserts every key as above, then adds the R
+// products into the value slots (LDS atomic adds).
+template <int LOG, int R, typename V>
+__device__ __forceinline__ void hash_accumulate_batch(int32_t *keys, V *vals, const int32_t (&key)[R],
+                                                      const V (&prod)[R], const bool (&ok)[R], int glane) {
+  constexpr uint32_t SH = 1u << LOG;
+  int32_t kk[R], old[R];
+  uint32_t h[R];
+#pragma unroll
+  for (int u = 0; u < R; u++) {
+    kk[u] = ok[u] ? key[u] : kDummy;
+    h[u] = ok[u] ? hash_slot<LOG>(key[u]) : SH + glane;
+  }
+#pragma unroll
+  for (int u = 0; u < R; u++) old[u] = atomicCAS(&keys[h[u]], kEmpty, kk[u]);
+#pragma unroll
+  for (int u = 0; u < R; u++) {
+    int ins = 0;
+    if (old[u] != kEmpty && old[u] != kk[u]) h[u] = hash_probe_from<LOG>(keys, kk[u], h[u], ins);
+  }
+#pragma unroll
+  for (int u = 0; u < R; u++)
+    __hip_atomic_fetch_add(&vals[h[u]], ok[u] ? prod[u] : (V)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---- group (sub-wave) collectives; GS lanes, GS a power of two <= 64 ------
@@ -70,9 +161,8 @@ template <int LOG>
 __device__ __forceinline__ int hash_insert_key(int32_t* keys, int32_t key) {
   constexpr uint32_t MASK = (1u << LOG) - 1;
   uint32_t h = hash_slot<LOG>(key);
-  volatile int32_t* vk = keys;
   while (true) {
-    int32_t k = vk[h];
+    int32_t k = lds_ld(&keys[h]);
     if (k == key) return 0;
     if (k == kEmpty) {
       int32_t old = atomicCAS(&keys[h], kEmpty, key);
@@ -88,9 +178,8 @@ template <int LOG, typename V>
 __device__ __forceinline__ void hash_accumulate(int32_t* keys, V* vals, int32_t key, V v) {
   constexpr uint32_t MASK = (1u << LOG) - 1;
   uint32_t h = hash_slot<LOG>(key);
-  volatile int32_t* vk = keys;
   while (true) {
-    int32_t k = vk[h];
+    int32_t k = lds_ld(&keys[h]);
     if (k == key) break;
     if (k == kEmpty) {
       int32_t old = atomicCAS(&keys[h], kEmpty, key);
@@ -106,7 +195,7 @@ template <int LOG, typename V>
 __device__ __forceinline__ V hash_lookup(const int32_t* keys, const V* vals, int32_t key) {
   constexpr uint32_t MASK = (1u << LOG) - 1;
   uint32_t h = hash_slot<LOG>(key);
-  while (keys[h] != key) h = (h + 1) & MASK;
+  for (uint32_t step = 0; step <= MASK && keys[h] != key; step++) h = (h + 1) & MASK;
   return vals[h];
 }
 
@@ -114,9 +203,8 @@ __device__ __forceinline__ V hash_lookup(const int32_t* keys, const V* vals, int
 __device__ __forceinline__ int hash_insert_key_rt(int32_t* keys, int32_t key, int log2sh) {
   const uint32_t mask = (1u << log2sh) - 1;
   uint32_t h = ((uint32_t)key * 0x9E3779B1u) >> (32 - log2sh);
-  volatile int32_t* vk = keys;
   while (true) {
-    int32_t k = vk[h];
+    int32_t k = lds_ld(&keys[h]);
     if (k == key) return 0;
     if (k == kEmpty) {
       int32_t old = atomicCAS(&keys[h], kEmpty, key);
@@ -124,6 +212,20 @@ __device__ __forceinline__ int hash_insert_key_rt(int32_t* keys, int32_t key, in
       if (old == key) return 0;
     }
     h = (h + 1) & mask;
+  }
+}
+
+// Lane exchange x -> lane ^ X for a constant X: DPP quad permutations for 1 and
+// 2, ds_swizzle (xor mode, within 32 lanes, no memory access) for 4..16,
+// ds_bpermute for 32.
+__device__ __forceinline__ int shfl_xor_const(int X, int v) {
+  switch (X) {
+    case 1: return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    case 2: return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    case 4: return __builtin_amdgcn_ds_swizzle(v, (4 << 10) | 0x1F);
+    case 8: return __builtin_amdgcn_ds_swizzle(v, (8 << 10) | 0x1F);
+    case 16: return __builtin_amdgcn_ds_swizzle(v, (16 << 10) | 0x1F);
+    default: return __shfl_xor(v, X, 64);
   }
 }
 
@@ -150,11 +252,10 @@ __device__ __forceinline__ void group_bitonic_sort(uint32_t (&v)[E], int glane) 
           }
         }
       } else {
-        const int lx = j / E;
 #pragma unroll
         for (int r = 0; r < E; r++) {
           const int g = glane * E + r;
-          uint32_t o = (uint32_t)__shfl_xor((int)v[r], lx, GS);
+          uint32_t o = (uint32_t)shfl_xor_const(j / E, (int)v[r]);
           const bool lower = (g & j) == 0;
           const bool asc = (g & k) == 0;
           uint32_t mn = v[r] < o ? v[r] : o;

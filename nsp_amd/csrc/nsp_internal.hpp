@@ -57,8 +57,9 @@ struct Handle {
   bool timing_pending = false;
   hipEvent_t events[8] = {};
 
-  // reusable device workspace
+  // reusable device workspace (ws: product; aux: compaction and gathers)
   Workspace ws;
+  Workspace aux;
   Info *d_info = nullptr;
   Info *h_info = nullptr;  // pinned
 

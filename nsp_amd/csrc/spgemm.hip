@@ -8,10 +8,11 @@
 // 1137-1300, numeric kernels 1308-2151 and their launcher 2178-2440, and the
 // driver nsp_spgemm_kernel_hash 2448-2488.
 //
-// Layout of the work: every kernel after binning walks a bin's row list
-// (row ids grouped by bin, order within a bin following row order per
-// wavefront) with a grid-stride loop, so a launch of any grid size covers
-// the bin; the host sizes grids from the bin counts it reads back.
+// Layout of the work: every hash kernel walks a list of rows with a
+// grid-stride loop, so a launch of any grid size covers it.  The list is a
+// range of bins of the binned row list, or all rows in natural order (direct
+// mode).  The host reads the bin counts back twice per product (after
+// binning, and with nnz(C) after the scan) and launches only what is needed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +20,8 @@
 
 #include "device_utils.hpp"
 #include "nsp_internal.hpp"
+#include "row_products.hpp"
+#include "scan.hpp"
 #include "spgemm_internal.hpp"
 
 namespace nsp {
@@ -51,6 +54,7 @@ __global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles) {
 // ---------------------------------------------------------------------------
 // Row info: intermediate products per row (nsp.cu:306-320), symbolic bin,
 // per-bin histogram, nprod, and int64 copies of A's and B's row pointers.
+// 16 lanes per row so the A.col reads are coalesced.
 template <typename RowT>
 __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
                                                  const int32_t *__restrict__ acol,
@@ -60,40 +64,40 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
                                                  int64_t *__restrict__ intprod,
                                                  int32_t *__restrict__ row_nz,
                                                  uint8_t *__restrict__ cls, Info *info) {
+  constexpr int GS = 16;
   __shared__ unsigned long long hist[kBins];
   __shared__ unsigned long long s_nprod, s_max;
   if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
   if (threadIdx.x == 0) { s_nprod = 0; s_max = 0; }
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int glane = threadIdx.x % GS;
   unsigned long long my_sum = 0, my_max = 0;
-  for (int64_t i = t0; i < MA; i += stride) {
+  for (int64_t i = blockIdx.x * (int64_t)(256 / GS) + threadIdx.x / GS; i < MA;
+       i += (int64_t)gridDim.x * (256 / GS)) {
     const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
     int64_t s = 0;
-    int64_t p = p0;
-    for (; p + 4 <= p1; p += 4) {
-      int32_t k0 = acol[p], k1 = acol[p + 1], k2 = acol[p + 2], k3 = acol[p + 3];
-      s += (int64_t)brpt[k0 + 1] - (int64_t)brpt[k0] + (int64_t)brpt[k1 + 1] - (int64_t)brpt[k1] +
-           (int64_t)brpt[k2 + 1] - (int64_t)brpt[k2] + (int64_t)brpt[k3 + 1] - (int64_t)brpt[k3];
-    }
-    for (; p < p1; p++) {
-      int32_t k = acol[p];
+    for (int64_t p = p0 + glane; p < p1; p += GS) {
+      const int32_t k = acol[p];
       s += (int64_t)brpt[k + 1] - (int64_t)brpt[k];
     }
-    intprod[i] = s;
-    arpt64[i] = p0;
-    if (s == 0) {
-      cls[i] = kNoBin;
-      row_nz[i] = 0;
-    } else {
-      int b = sym_bin_of(s);
-      cls[i] = (uint8_t)b;
-      atomicAdd(&hist[b], 1ull);
+    s = group_sum<GS>(s);
+    if (glane == 0) {
+      intprod[i] = s;
+      arpt64[i] = p0;
+      if (s == 0) {
+        cls[i] = kNoBin;
+        row_nz[i] = 0;
+      } else {
+        const int b = sym_bin_of(s);
+        cls[i] = (uint8_t)b;
+        atomicAdd(&hist[b], 1ull);
+      }
+      my_sum += (unsigned long long)s;
+      my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
     }
-    my_sum += (unsigned long long)s;
-    my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
   }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   for (int64_t j = t0; j <= MB; j += stride) brpt64[j] = (int64_t)brpt[j];
   if (t0 == 0) arpt64[MA] = (int64_t)arpt[MA];
   my_sum = group_sum<64>(my_sum);
@@ -112,36 +116,58 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
 }
 
 // ---------------------------------------------------------------------------
-// Scatter rows into per-bin lists.  Each wavefront claims one slot range per
-// bin with a single atomic, so rows of a bin stay in row order within a wave.
-__global__ __launch_bounds__(256) void k_scatter(int64_t M, const uint8_t *__restrict__ cls,
+// Scatter rows into per-bin lists, stably within a block's chunk of rows:
+// pass 1 counts the chunk's rows per bin and claims one range per bin with a
+// single global atomic; pass 2 places rows in order using wavefront ballots.
+__global__ __launch_bounds__(256) void k_scatter(int64_t M, int64_t chunk,
+                                                 const uint8_t *__restrict__ cls,
                                                  const int64_t *__restrict__ counts,
                                                  int64_t *__restrict__ cursor,
                                                  int32_t *__restrict__ perm) {
   __shared__ int64_t off[kBins];
-  if (threadIdx.x == 0) {
-    int64_t s = 0;
-    for (int b = 0; b < kBins; b++) { off[b] = s; s += counts[b]; }
+  __shared__ int64_t run[kBins];
+  __shared__ unsigned int hist[kBins];
+  __shared__ int wcnt[4][kBins];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1;
+  const int64_t c0 = blockIdx.x * chunk;
+  const int64_t c1 = c0 + chunk < M ? c0 + chunk : M;
+  if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t i = c0 + threadIdx.x; i < c1; i += 256) {
+    const int c = cls[i];
+    if (c != kNoBin) atomicAdd(&hist[c], 1u);
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  // all lanes iterate together (uniform trip count per wave) for the ballots
-  for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < M;
-       base += stride) {
-    const int64_t i = base + lane;
-    const int c = (i < M) ? (int)cls[i] : (int)kNoBin;
+  if (threadIdx.x < kBins) {
+    int64_t s = 0;
+    for (int b = 0; b < (int)threadIdx.x; b++) s += counts[b];
+    off[threadIdx.x] = s;
+    const unsigned int hcount = hist[threadIdx.x];
+    run[threadIdx.x] =
+        hcount ? (int64_t)atomicAdd((unsigned long long *)&cursor[threadIdx.x], (unsigned long long)hcount) : 0;
+  }
+  __syncthreads();
+  for (int64_t i0 = c0; i0 < c1; i0 += 256) {
+    const int64_t i = i0 + threadIdx.x;
+    const int c = (i < c1) ? (int)cls[i] : (int)kNoBin;
+    int myrank = 0;
 #pragma unroll
     for (int b = 0; b < kBins; b++) {
       const uint64_t m = __ballot(c == b);
-      if (m == 0) continue;
-      int64_t pos0 = 0;
-      if (lane == __ffsll((unsigned long long)m) - 1)
-        pos0 = (int64_t)atomicAdd((unsigned long long *)&cursor[b], (unsigned long long)__popcll(m));
-      pos0 = __shfl(pos0, __ffsll((unsigned long long)m) - 1);
-      if (c == b) perm[off[b] + pos0 + __popcll(m & lt)] = (int32_t)i;
+      if (lane == 0) wcnt[w][b] = __popcll(m);
+      if (c == b) myrank = __popcll(m & lt);
     }
+    __syncthreads();
+    if (c != kNoBin) {
+      int64_t pos = run[c] + myrank;
+      for (int q = 0; q < w; q++) pos += wcnt[q][c];
+      perm[off[c] + pos] = (int32_t)i;
+    }
+    __syncthreads();
+    if (threadIdx.x < kBins) run[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] +
+                                                  wcnt[2][threadIdx.x] + wcnt[3][threadIdx.x];
+    __syncthreads();
   }
 }
 
@@ -163,10 +189,20 @@ __device__ __forceinline__ void bin_range(const int64_t *counts, int lo, int hi,
   for (int q = lo; q <= hi; q++) count += counts[q];
 }
 
+// Clears GS lanes' share of an LDS table with 16-byte stores.
+template <int GS, int N, typename T>
+__device__ __forceinline__ void clear_lds(T *p, int glane, int32_t fill) {
+  static_assert((N * sizeof(T)) % 16 == 0, "table size");
+  int4 *q = reinterpret_cast<int4 *>(p);
+  constexpr int NV = (int)(N * sizeof(T) / 16);
+  const int4 f = make_int4(fill, fill, fill, fill);
+  for (int s = glane; s < NV; s += GS) q[s] = f;
+}
+
 // ---------------------------------------------------------------------------
 // Symbolic, GS lanes per row (sub-wave groups or one wave), SH-slot LDS key
-// table per row, SUB lanes per A entry.  256 threads per block.
-template <int GS, int SUB, int SH>
+// table per row; products enumerated with R loads in flight per lane.
+template <int GS, int SH, int R>
 __global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
                                                    const Info *__restrict__ info,
                                                    const int32_t *__restrict__ perm,
@@ -177,8 +213,9 @@ __global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64
                                                    int32_t *__restrict__ row_nz) {
   constexpr int RPB = 256 / GS;
   constexpr int LOG = Log2<SH>::value;
-  constexpr int NSUB = GS / SUB;
-  __shared__ int32_t keys[RPB][SH];
+  constexpr int SHD = SH + (GS < 16 ? 16 : GS);  // + one dummy slot per lane
+  __shared__ int32_t keys[RPB][SHD];
+  __shared__ EntryScratch<GS, R, float> ent[RPB];
   int64_t count, off;
   bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
@@ -187,16 +224,14 @@ __global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64
     const int64_t t = base + slot;
     if (t < count) {
       const int32_t r = perm ? perm[off + t] : (int32_t)t;
-#pragma unroll
-      for (int s = glane; s < SH; s += GS) K[s] = kEmpty;
+      clear_lds<GS, SHD>(K, glane, kEmpty);
       wave_lds_sync();
-      const int64_t a0 = arpt[r], a1 = arpt[r + 1];
       int cnt = 0;
-      for (int64_t ja = a0 + glane / SUB; ja < a1; ja += NSUB) {
-        const int32_t k = acol[ja];
-        const int64_t b0 = brpt[k], b1 = brpt[k + 1];
-        for (int64_t jb = b0 + glane % SUB; jb < b1; jb += SUB) cnt += hash_insert_key<LOG>(K, bcol[jb]);
-      }
+      row_products<GS, R, false, float>(
+          glane, arpt[r], arpt[r + 1], 0, GS, acol, nullptr, brpt, bcol, nullptr, &ent[slot],
+          [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R]) {
+            cnt += hash_insert_keys<LOG, R>(K, key, ok, glane);
+          });
       cnt = group_sum<GS>(cnt);
       if (glane == 0) row_nz[r] = cnt;
       wave_lds_sync();
@@ -204,8 +239,9 @@ __global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64
   }
 }
 
-// Symbolic, one block of BS threads per row.
-template <int BS, int SUB, int SH>
+// Symbolic, one block of BS threads per row; the row's A entries are split
+// over the block's waves in chunks of 64.
+template <int BS, int SH, int R>
 __global__ __launch_bounds__(BS) void k_sym_block(int bin_lo, int bin_hi, int64_t direct_rows,
                                                   const Info *__restrict__ info,
                                                   const int32_t *__restrict__ perm,
@@ -215,28 +251,29 @@ __global__ __launch_bounds__(BS) void k_sym_block(int bin_lo, int bin_hi, int64_
                                                   const int32_t *__restrict__ bcol,
                                                   int32_t *__restrict__ row_nz) {
   constexpr int LOG = Log2<SH>::value;
-  constexpr int NSUB = BS / SUB;
-  __shared__ int32_t K[SH];
-  __shared__ int red[BS / 64];
+  constexpr int NW = BS / 64;
+  __shared__ int32_t K[SH + 64];
+  __shared__ EntryScratch<64, R, float> ent[NW];
+  __shared__ int red[NW];
   int64_t count, off;
   bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int64_t t = blockIdx.x; t < count; t += gridDim.x) {
     const int32_t r = perm ? perm[off + t] : (int32_t)t;
-    for (int s = threadIdx.x; s < SH; s += BS) K[s] = kEmpty;
+    clear_lds<BS, SH + 64>(K, threadIdx.x, kEmpty);
     __syncthreads();
-    const int64_t a0 = arpt[r], a1 = arpt[r + 1];
     int cnt = 0;
-    for (int64_t ja = a0 + threadIdx.x / SUB; ja < a1; ja += NSUB) {
-      const int32_t k = acol[ja];
-      const int64_t b0 = brpt[k], b1 = brpt[k + 1];
-      for (int64_t jb = b0 + threadIdx.x % SUB; jb < b1; jb += SUB) cnt += hash_insert_key<LOG>(K, bcol[jb]);
-    }
+    row_products<64, R, false, float>(
+        lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, acol, nullptr, brpt, bcol, nullptr, &ent[w],
+        [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R]) {
+          cnt += hash_insert_keys<LOG, R>(K, key, ok, lane);
+        });
     cnt = group_sum<64>(cnt);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    if (lane == 0) red[w] = cnt;
     __syncthreads();
     if (threadIdx.x == 0) {
       int s = 0;
-      for (int w = 0; w < BS / 64; w++) s += red[w];
+      for (int q = 0; q < NW; q++) s += red[q];
       row_nz[r] = s;
     }
     __syncthreads();
@@ -244,20 +281,8 @@ __global__ __launch_bounds__(BS) void k_sym_block(int bin_lo, int bin_hi, int64_
 }
 
 // ---------------------------------------------------------------------------
-// Exclusive scan of row_nz into C's row pointer: single pass, tiles claimed
-// in order through a ticket, decoupled look-back by one wavefront over 64
-// predecessor status words at a time.  Status word = (value << 2) | flag,
-// flag 1 = tile aggregate, 2 = inclusive prefix.  Also bins rows by nnz for
-// the numeric pass.
-constexpr int kScanBS = 256, kScanItems = 8, kScanTile = kScanBS * kScanItems;
-
-__device__ __forceinline__ uint64_t ld_status(const uint64_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_status(uint64_t *p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
+// Exclusive scan of row_nz into C's row pointer (single pass, decoupled
+// look-back; scan.hpp), fused with the numeric binning of rows by nnz.
 template <typename RowT>
 __global__ __launch_bounds__(kScanBS) void k_scan(int64_t M, const int32_t *__restrict__ row_nz,
                                                   RowT *__restrict__ crpt,
@@ -298,44 +323,13 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int64_t M, const int32_t *__re
   }
   int64_t agg;
   int64_t texcl = block_excl_scan<kScanBS, int64_t>(tsum, s_wsum, &agg);
-  // look-back
-  if (threadIdx.x < 64) {
-    const int lane = threadIdx.x;
-    if (tile == 0) {
-      if (lane == 0) {
-        st_status(&status[0], ((uint64_t)agg << 2) | 2u);
-        s_prefix = 0;
-      }
-    } else {
-      if (lane == 0) st_status(&status[tile], ((uint64_t)agg << 2) | 1u);
-      int64_t excl = 0;
-      int64_t j = tile - 1;
-      while (true) {
-        const int64_t idx = j - lane;
-        uint64_t s = (idx >= 0) ? ld_status(&status[idx]) : 2u;
-        while ((s & 3u) == 0) s = ld_status(&status[idx]);
-        const uint64_t incl = __ballot((s & 3u) == 2u);
-        const int64_t val = (int64_t)(s >> 2);
-        if (incl) {
-          const int first = __ffsll((unsigned long long)incl) - 1;
-          excl += group_sum<64>(lane <= first ? val : (int64_t)0);
-          break;
-        }
-        excl += group_sum<64>(val);
-        j -= 64;
-      }
-      if (lane == 0) {
-        st_status(&status[tile], ((uint64_t)(excl + agg) << 2) | 2u);
-        s_prefix = excl;
-      }
-    }
-  }
+  const int64_t prefix = scan_lookback(tile, agg, status, &s_prefix);
   tmax = group_max<64>(tmax);
   if ((threadIdx.x & 63) == 0) atomicMax(&s_max, tmax);
   __syncthreads();
   // The grid has floor(M / tile) + 1 tiles, so exactly one thread holds
   // index M (the end of C's row pointer, i.e. nnz(C)).
-  int64_t run = s_prefix + texcl;
+  int64_t run = prefix + texcl;
 #pragma unroll
   for (int q = 0; q < kScanItems; q++) {
     const int64_t i = base + q;
@@ -358,7 +352,7 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int64_t M, const int32_t *__re
 // Numeric, GS lanes per row, SH-slot LDS table (keys + values) per row, then
 // compaction of the occupied slots, a register bitonic sort of the row's
 // columns (GS lanes x E keys) and a lookup of each sorted column's value.
-template <typename V, int GS, int SUB, int SH>
+template <typename V, int GS, int NS, int SHF, int R>
 __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64_t direct_rows,
                                                    const Info *__restrict__ info,
                                                    const int32_t *__restrict__ perm,
@@ -372,15 +366,16 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
                                                    int32_t *__restrict__ ccol,
                                                    V *__restrict__ cval) {
   constexpr int RPB = 256 / GS;
+  constexpr int SH = NS * SHF;    // table slots (NS = largest row nnz of the bin)
   constexpr int LOG = Log2<SH>::value;
-  constexpr int NSUB = GS / SUB;
-  constexpr int NS = SH / 2;      // sort width
+  constexpr int SHD = SH + GS;    // + one dummy slot per lane
   constexpr int E = NS / GS;      // keys per lane in the sort
   constexpr int SPL = SH / GS;    // table slots per lane in the compaction
   static_assert(E >= 1, "sort width below group size");
-  __shared__ int32_t keys[RPB][SH];
-  __shared__ V vals[RPB][SH];
+  __shared__ int32_t keys[RPB][SHD];
+  __shared__ V vals[RPB][SHD];
   __shared__ uint32_t stage[RPB][NS];
+  __shared__ EntryScratch<GS, R, V> ent[RPB];
   int64_t count, off;
   bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
@@ -391,20 +386,14 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
     const int64_t t = base + slot;
     if (t < count) {
       const int32_t r = perm ? perm[off + t] : (int32_t)t;
-#pragma unroll
-      for (int s = glane; s < SH; s += GS) {
-        K[s] = kEmpty;
-        W[s] = (V)0;
-      }
+      clear_lds<GS, SHD>(K, glane, kEmpty);
+      clear_lds<GS, SHD>(W, glane, 0);
       wave_lds_sync();
-      const int64_t a0 = arpt[r], a1 = arpt[r + 1];
-      for (int64_t ja = a0 + glane / SUB; ja < a1; ja += NSUB) {
-        const int32_t k = acol[ja];
-        const V av = aval[ja];
-        const int64_t b0 = brpt[k], b1 = brpt[k + 1];
-        for (int64_t jb = b0 + glane % SUB; jb < b1; jb += SUB)
-          hash_accumulate<LOG, V>(K, W, bcol[jb], av * bval[jb]);
-      }
+      row_products<GS, R, true, V>(glane, arpt[r], arpt[r + 1], 0, GS, acol, aval, brpt, bcol, bval,
+                                   &ent[slot],
+                                   [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
+                                     hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, glane);
+                                   });
       wave_lds_sync();
       // compact occupied slots (strided: slot q*GS + glane) into S
       int c = 0;
@@ -440,7 +429,7 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
 }
 
 // Numeric, one block of BS threads per row; table and sort in LDS.
-template <typename V, int BS, int SUB, int SH>
+template <typename V, int BS, int NS, int SHF, int R>
 __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_t direct_rows,
                                                   const Info *__restrict__ info,
                                                   const int32_t *__restrict__ perm,
@@ -453,30 +442,27 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
                                                   const int64_t *__restrict__ crpt,
                                                   int32_t *__restrict__ ccol,
                                                   V *__restrict__ cval) {
+  constexpr int SH = NS * SHF;
   constexpr int LOG = Log2<SH>::value;
-  constexpr int NSUB = BS / SUB;
-  constexpr int NS = SH / 2;
-  __shared__ int32_t K[SH];
-  __shared__ V W[SH];
+  constexpr int NW = BS / 64;
+  __shared__ int32_t K[SH + 64];
+  __shared__ V W[SH + 64];
   __shared__ uint32_t S[NS];
-  __shared__ int scan_tmp[BS / 64 + 1];
+  __shared__ EntryScratch<64, R, V> ent[NW];
+  __shared__ int scan_tmp[NW + 1];
   int64_t count, off;
   bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int64_t t = blockIdx.x; t < count; t += gridDim.x) {
     const int32_t r = perm ? perm[off + t] : (int32_t)t;
-    for (int s = threadIdx.x; s < SH; s += BS) {
-      K[s] = kEmpty;
-      W[s] = (V)0;
-    }
+    clear_lds<BS, SH + 64>(K, threadIdx.x, kEmpty);
+    clear_lds<BS, SH + 64>(W, threadIdx.x, 0);
     __syncthreads();
-    const int64_t a0 = arpt[r], a1 = arpt[r + 1];
-    for (int64_t ja = a0 + threadIdx.x / SUB; ja < a1; ja += NSUB) {
-      const int32_t k = acol[ja];
-      const V av = aval[ja];
-      const int64_t b0 = brpt[k], b1 = brpt[k + 1];
-      for (int64_t jb = b0 + threadIdx.x % SUB; jb < b1; jb += SUB)
-        hash_accumulate<LOG, V>(K, W, bcol[jb], av * bval[jb]);
-    }
+    row_products<64, R, true, V>(lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, acol, aval, brpt, bcol, bval,
+                                 &ent[w],
+                                 [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
+                                   hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, lane);
+                                 });
     __syncthreads();
     constexpr int SPT = SH / BS;
     int c = 0;
@@ -504,10 +490,6 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
 
 // ---------------------------------------------------------------------------
 // host-side launch tables
-struct GridCfg {
-  int blocks;
-};
-
 static int grid_for(const Handle *h, int64_t items, int per_block, int blocks_per_cu) {
   int64_t need = (items + per_block - 1) / per_block;
   int64_t cap = (int64_t)h->num_cu * blocks_per_cu;
@@ -515,16 +497,12 @@ static int grid_for(const Handle *h, int64_t items, int per_block, int blocks_pe
   return (int)std::min(need, cap);
 }
 
-#define NSP_SUB_SWITCH(sub, ...)                                  \
-  switch (sub) {                                                  \
-    case 1: { constexpr int SUB_ = 1; __VA_ARGS__; } break;       \
-    case 4: { constexpr int SUB_ = 4; __VA_ARGS__; } break;       \
-    default: { constexpr int SUB_ = 16; __VA_ARGS__; } break;     \
-  }
+constexpr int kSymR = 8;  // B loads in flight per lane, symbolic
+constexpr int kNumR = 4;  // B loads in flight per lane, numeric
 
 // Launches the kernel of bin `bin` over bins [lo, bin] (rows of lower bins fit
 // its larger table), or over all rows in natural order when direct >= 0.
-static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int sub, int64_t count,
+static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
                            const int64_t *arpt, const int32_t *acol, const int64_t *brpt,
                            const int32_t *bcol) {
   hipStream_t st = h->stream;
@@ -532,20 +510,12 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int sub, 
   const int32_t *perm = direct >= 0 ? nullptr : h->perm_sym;
   int32_t *rnz = h->row_nz;
   const int cap = h->grid_cap;
-#define SYM_GROUP(GS, SH)                                                                        \
-  do {                                                                                           \
-    int g = grid_for(h, count, 256 / GS, cap);                                                   \
-    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_sym_group<GS, (SUB_ > GS ? GS : SUB_), SH>),       \
-                                           dim3(g), dim3(256), 0, st, lo, bin, direct, info,     \
-                                           perm, arpt, acol, brpt, bcol, rnz));                              \
-  } while (0)
-#define SYM_BLOCK(BS, SH)                                                                        \
-  do {                                                                                           \
-    int g = grid_for(h, count, 1, cap);                                                          \
-    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_sym_block<BS, SUB_, SH>), dim3(g), dim3(BS), 0,    \
-                                           st, lo, bin, direct, info, perm, arpt, acol, brpt,   \
-                                           bcol, rnz));                                          \
-  } while (0)
+#define SYM_GROUP(GS, SH)                                                                          \
+  hipLaunchKernelGGL((k_sym_group<GS, SH, kSymR>), dim3(grid_for(h, count, 256 / GS, cap)),        \
+                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, acol, brpt, bcol, rnz)
+#define SYM_BLOCK(BS, SH)                                                                          \
+  hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, cap)), dim3(BS), 0, \
+                     st, lo, bin, direct, info, perm, arpt, acol, brpt, bcol, rnz)
   switch (bin) {
     case 0: SYM_GROUP(16, 32); break;
     case 1: SYM_GROUP(32, 64); break;
@@ -562,7 +532,7 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int sub, 
 }
 
 template <typename V>
-static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int sub, int64_t count,
+static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
                            const int64_t *arpt, const int32_t *acol, const V *aval,
                            const int64_t *brpt, const int32_t *bcol, const V *bval,
                            int32_t *ccol, V *cval) {
@@ -571,30 +541,25 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int sub, 
   const int32_t *perm = direct >= 0 ? nullptr : h->perm_num;
   const int64_t *crpt = h->c_rpt64;
   const int cap = h->grid_cap;
-#define NUM_GROUP(GS, SH)                                                                         \
-  do {                                                                                            \
-    int g = grid_for(h, count, 256 / GS, cap);                                                    \
-    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_num_group<V, GS, (SUB_ > GS ? GS : SUB_), SH>),     \
-                                           dim3(g), dim3(256), 0, st, lo, bin, direct, info,      \
-                                           perm, arpt, acol, aval, brpt, bcol, bval, crpt, ccol,  \
-                                           cval));                                                \
-  } while (0)
-#define NUM_BLOCK(BS, SH)                                                                         \
-  do {                                                                                            \
-    int g = grid_for(h, count, 1, cap);                                                           \
-    NSP_SUB_SWITCH(sub, hipLaunchKernelGGL((k_num_block<V, BS, SUB_, SH>), dim3(g), dim3(BS), 0,  \
-                                           st, lo, bin, direct, info, perm, arpt, acol, aval,     \
-                                           brpt, bcol, bval, crpt, ccol, cval));                  \
-  } while (0)
+#define NUM_GROUP(GS, NS, SHF)                                                                      \
+  hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR>), dim3(grid_for(h, count, 256 / GS, cap)),      \
+                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, acol, aval, brpt, bcol,   \
+                     bval, crpt, ccol, cval)
+#define NUM_BLOCK(BS, NS, SHF)                                                                      \
+  hipLaunchKernelGGL((k_num_block<V, BS, NS, SHF, kNumR>), dim3(grid_for(h, count, 1, cap)), dim3(BS),   \
+                     0, st, lo, bin, direct, info, perm, arpt, acol, aval, brpt, bcol, bval, crpt,  \
+                     ccol, cval)
   switch (bin) {
-    case 0: NUM_GROUP(16, 32); break;
-    case 1: NUM_GROUP(32, 64); break;
-    case 2: NUM_GROUP(64, 128); break;
-    case 3: NUM_GROUP(64, 256); break;
-    case 4: NUM_GROUP(64, 512); break;
-    case 5: NUM_GROUP(64, 1024); break;
-    case 6: NUM_BLOCK(256, 4096); break;
-    case 7: NUM_BLOCK(512, 8192); break;
+    // table factor 4 for the small bins (near-collision-free probes), 2 where
+    // the LDS footprint would limit occupancy
+    case 0: NUM_GROUP(16, 16, 4); break;
+    case 1: NUM_GROUP(32, 32, 4); break;
+    case 2: NUM_GROUP(64, 64, 4); break;
+    case 3: NUM_GROUP(64, 128, 4); break;
+    case 4: NUM_GROUP(64, 256, 2); break;
+    case 5: NUM_GROUP(64, 512, 2); break;
+    case 6: NUM_BLOCK(256, 2048, 2); break;
+    case 7: NUM_BLOCK(512, 4096, 2); break;
     default: break;
   }
 #undef NUM_GROUP
@@ -632,6 +597,22 @@ static int plan_launches(const int64_t *counts, int64_t M, Launch *out) {
   for (int b = kBins - 1; b >= 6; b--)
     if (counts[b] > 0) out[n++] = Launch{b, b, -1, counts[b]};
   return n;
+}
+
+static bool needs_row_list(const Launch *L, int n) {
+  for (int q = 0; q < n; q++)
+    if (L[q].direct < 0) return true;
+  return false;
+}
+
+static void launch_scatter(Handle *h, int64_t M, int64_t *counts, int64_t *cursor, int32_t *perm) {
+  if (M <= 0) return;
+  const int64_t g = std::max<int64_t>(1, std::min<int64_t>((M + 2047) / 2048, 2048));
+  int64_t chunk = (M + g - 1) / g;
+  chunk = (chunk + 255) / 256 * 256;
+  const int64_t blocks = (M + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)blocks), dim3(256), 0, h->stream, M, chunk, h->cls, counts,
+                     cursor, perm);
 }
 
 // Optional per-phase device timing with HIP events on the handle's stream:
@@ -698,14 +679,6 @@ static int check_csr_args(Handle *h, const char *name, const nsp_csr_t *X) {
   return NSP_OK;
 }
 
-static int pick_sub(const Handle *h, const nsp_csr_t *B) {
-  if (h->sub_lanes == 1 || h->sub_lanes == 4 || h->sub_lanes == 16) return h->sub_lanes;
-  const double avg = B->rows > 0 ? (double)B->nnz / (double)B->rows : 0.0;
-  if (avg < 2.0) return 1;
-  if (avg < 9.0) return 4;
-  return 16;
-}
-
 static int read_info(Handle *h) {
   NSP_HIP(h, hipMemcpyAsync(h->h_info, h->d_info, sizeof(Info), hipMemcpyDeviceToHost, h->stream));
   NSP_HIP(h, hipStreamSynchronize(h->stream));
@@ -713,7 +686,7 @@ static int read_info(Handle *h) {
   return NSP_OK;
 }
 
-// Row info + symbolic binning; leaves h_info filled.  Also used by nsp_intprod.
+// Row info + symbolic binning (no host sync).  Also used by nsp_intprod.
 template <typename RowT>
 static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B) {
   const int64_t MA = A->rows, MB = B->rows;
@@ -721,7 +694,7 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B) {
   timing_mark(h, 0);
   hipLaunchKernelGGL(k_init, dim3(std::max<int64_t>(1, std::min<int64_t>((h->ntiles + 255) / 256, 1024))),
                      dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles);
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((std::max(MA, MB + 1) + 255) / 256,
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(std::max((MA + 15) / 16, (MB + 256) / 256),
                                                             (int64_t)h->num_cu * 8));
   hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr,
                      A->col, (const RowT *)B->row_ptr, MB, h->a_rpt64, h->b_rpt64, h->intprod,
@@ -738,24 +711,19 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
   h->have_plan = false;
   h->timing_pending = false;
   NSP_TRY(run_rowinfo<RowT>(h, A, B));
-  if (MA > 0) {
-    const int gs = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 255) / 256, (int64_t)h->num_cu * 8));
-    hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(256), 0, h->stream, MA, h->cls,
-                       h->d_info->count_sym, h->d_info->cursor_sym, h->perm_sym);
-  }
   NSP_TRY(read_info(h));
   const Info &hi = *h->h_info;
   if (hi.count_sym[kHeavyBin] > 0)
     return set_error(h, NSP_ERR_UNSUPPORTED,
                      "rows with more than 16384 intermediate products need the column-window path");
-  const int sub = pick_sub(h, B);
-  timing_mark(h, 1);
   {
     Launch L[kBins];
     const int n = plan_launches(hi.count_sym, MA, L);
+    if (needs_row_list(L, n))
+      launch_scatter(h, MA, h->d_info->count_sym, h->d_info->cursor_sym, h->perm_sym);
+    timing_mark(h, 1);
     for (int q = 0; q < n; q++)
-      launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, sub, L[q].count, h->a_rpt64, A->col,
-                     h->b_rpt64, B->col);
+      launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, A->col, h->b_rpt64, B->col);
     h->stats.sym_launches = n;
   }
   timing_mark(h, 2);
@@ -764,17 +732,13 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
     int64_t ntile_used = MA / kScanTile + 1;
     hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)ntile_used), dim3(kScanBS), 0, h->stream, MA,
                        h->row_nz, (RowT *)c_row_ptr, h->c_rpt64, h->cls, h->tile_status, h->d_info);
-    const int gs = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 255) / 256, (int64_t)h->num_cu * 8));
-    hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(256), 0, h->stream, MA, h->cls,
-                       h->d_info->count_num, h->d_info->cursor_num, h->perm_num);
-    timing_mark(h, 3);
   } else {
-    timing_mark(h, 3);
     RowT zero = 0;
     NSP_HIP(h, hipMemcpyAsync(c_row_ptr, &zero, sizeof(RowT), hipMemcpyHostToDevice, h->stream));
     int64_t z64 = 0;
     NSP_HIP(h, hipMemcpyAsync(h->c_rpt64, &z64, 8, hipMemcpyHostToDevice, h->stream));
   }
+  timing_mark(h, 3);
   NSP_HIP(h, hipGetLastError());
   NSP_TRY(read_info(h));
   if (hi.error) return set_error(h, (int)hi.error, "C has 2^31 or more entries; use NSP_IDX64");
@@ -784,7 +748,7 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
   h->stats.max_row_nnz = hi.max_nz;
   h->stats.n_sym_bins = kBins;
   h->stats.n_num_bins = kBins;
-  h->stats.sub_lanes = sub;
+  h->stats.sub_lanes = 0;
   for (int b = 0; b < kBins; b++) {
     h->stats.sym_bin[b] = hi.count_sym[b];
     h->stats.num_bin[b] = hi.count_num[b];
@@ -798,7 +762,6 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
   h->plan_nnz = h->stats.nnz;
   h->plan_a = A->col;
   h->plan_b = B->col;
-  h->plan_sub = sub;
   return NSP_OK;
 }
 
@@ -820,12 +783,14 @@ int spgemm_symbolic(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *
 
 template <typename V>
 static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C) {
-  timing_mark(h, 4);
   Launch L[kBins];
   const int n = plan_launches(h->num_count, A->rows, L);
+  if (needs_row_list(L, n))
+    launch_scatter(h, A->rows, h->d_info->count_num, h->d_info->cursor_num, h->perm_num);
+  timing_mark(h, 4);
   for (int q = 0; q < n; q++)
-    launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, h->plan_sub, L[q].count, h->a_rpt64, A->col,
-                      (const V *)A->val, h->b_rpt64, B->col, (const V *)B->val, C->col, (V *)C->val);
+    launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, A->col, (const V *)A->val,
+                      h->b_rpt64, B->col, (const V *)B->val, C->col, (V *)C->val);
   h->stats.num_launches = n;
   timing_mark(h, 5);
   h->timing_pending = h->timing;

@@ -10,4 +10,10 @@ int spgemm_numeric(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A
 int spgemm_intprod(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, int64_t *intprod,
                    int64_t *nprod);
 int check_csr(Handle *h, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row);
+int shrink_cols(Handle *h, int64_t nnz, const int32_t *col, int64_t l, int64_t lo, int64_t hi,
+                int32_t *new_col, int32_t *perm, int64_t perm_cap, int64_t *shrinked_m);
+int gather_rows_count(Handle *h, nsp_index_t it, const nsp_csr_t *B, int64_t nr, const int32_t *rows,
+                      int64_t row_offset, void *out_rpt, int64_t *out_nnz);
+int gather_rows_fill(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *B, int64_t nr,
+                     const int32_t *rows, int64_t row_offset, int64_t col_shift, nsp_csr_t *out);
 }  // namespace nsp

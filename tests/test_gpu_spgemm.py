@@ -156,3 +156,50 @@ def test_check_csr(nsp):
     r = int(np.argmax(np.diff(bad.rpt) >= 2))
     bad.col[bad.rpt[r]], bad.col[bad.rpt[r] + 1] = bad.col[bad.rpt[r] + 1], bad.col[bad.rpt[r]]
     assert h.check_csr(nsp.DeviceCSR.from_host(bad)) == r
+
+
+# ---------------------------------------------------------------- local share
+@pytest.mark.parametrize("nprocs,rank", [(4, 0), (3, 1), (3, 2)])
+def test_local_share_matches_oracle(nsp, orc, nprocs, rank):
+    """One rank's share: compaction maps, completedP, and the compacted product
+    equal the oracle's restatement of nsparseMGPU_version1."""
+    print("case: local share", nprocs, rank, flush=True)
+    from nsp_amd import csr
+    A = mixed_rows(600, 600, 1, 12, seed=40 + rank)
+    r0, r1 = csr.rank_rows(A.nrows, nprocs, rank)
+    A_rank = A.row_slice(r0, r1, col_shift=r0)
+    ref = orc.local_share(A_rank, A, r0)
+    dAr = nsp.DeviceCSR.from_host(A_rank)
+    Al = csr.CSR.from_device(dAr, full_n=A.nrows, row_shift=r0)
+    Al.m = A.ncols
+    Pfull = nsp.DeviceCSR.from_host(A)
+    C, A_, P_, _ = csr.nsparse_mgpu_version1_local(Al, Pfull)
+    np.testing.assert_array_equal(A_.col.cpu().numpy(), ref["A_"].col)
+    assert A_.m == ref["A_"].ncols
+    np.testing.assert_array_equal(P_.row.cpu().numpy().astype(np.int64), ref["P_"].rpt)
+    np.testing.assert_array_equal(P_.col.cpu().numpy(), ref["P_"].col)
+    np.testing.assert_array_equal(P_.val.cpu().numpy(), ref["P_"].val)
+    np.testing.assert_array_equal(C.extra["shrinking_permut_P"].cpu().numpy(), ref["perm_P"])
+    assert P_.m == ref["P_"].ncols
+    check_against_oracle(ref["A_"], ref["P_"], nsp.DeviceCSR(C.row, C.col, C.val, C.n, C.m), "f64", orc)
+
+
+def test_stencil_rank0_share_identity(nsp, orc):
+    """27-point stencil, rank 0 of 4: both compaction maps are the identity and
+    the product equals the direct block product (m=32)."""
+    print("case: stencil share identity", flush=True)
+    from nsp_amd import csr
+    m = 32
+    n = m ** 3
+    A = gen.stencil_27pt(m, seed=2)
+    A0 = A.row_slice(0, n // 4)
+    Al = csr.CSR.from_device(nsp.DeviceCSR.from_host(A0), full_n=n, row_shift=0)
+    Al.m = n
+    C, A_, P_, _ = csr.nsparse_mgpu_version1_local(Al, nsp.DeviceCSR.from_host(A))
+    assert A_.m == n // 4 + m * m and P_.m == n // 4 + 2 * m * m
+    np.testing.assert_array_equal(A_.col.cpu().numpy(), A0.col)
+    np.testing.assert_array_equal(C.extra["shrinking_permut_P"].cpu().numpy(), np.arange(P_.m))
+    ref = orc.spgemm(A0, A)
+    crpt, ccol, cval, cabs = ref
+    np.testing.assert_array_equal(C.row.cpu().numpy().astype(np.int64), crpt)
+    np.testing.assert_array_equal(C.col.cpu().numpy(), ccol)
