@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--param", action="append", default=[],
+                    help="library tuning parameter name=value (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -118,6 +120,11 @@ def main():
     dB = nsp_amd.DeviceCSR.from_host(B, "f64", "i32")
     h = nsp_amd.Handle()
     h.set_param("timing", 1)
+    params = {}
+    for kv in args.param:
+        k, v = kv.split("=")
+        h.set_param(k, int(v))
+        params[k] = int(v)
 
     rpt, nnz = h.symbolic(dA, dB)
     C = h.numeric(dA, dB, rpt, nnz)
@@ -168,6 +175,7 @@ def main():
         "two_pass_bytes": sym_b + scan_b + num_b,
         "two_pass_GBs_per_step": round((sym_b + scan_b + num_b) / (ms / 1e3) / 1e9, 1),
         "host_syncs_per_step": st["host_syncs"],
+        "params": params,
         "bins": {"sym": st["sym_bin"], "num": st["num_bin"], "sym_launches": st["sym_launches"],
                  "num_launches": st["num_launches"], "sub_lanes": st["sub_lanes"]},
     }

@@ -90,6 +90,11 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *                  length of B, else 1, 4 or 16)
  *   "sym_factor"   symbolic hash table slots per intermediate product (1, 2)
  *   "grid_cap"     blocks per CU for persistent launches
+ *   "sym_rows_per_wave"  rows per wavefront in the symbolic bins of 129..1024
+ *                  products (1, 2, 4)
+ *   "num_rows_per_wave"  rows per wavefront in the numeric bins of 65..256
+ *                  entries (1, 2)
+ *   "num_table_factor"   numeric hash slots per row entry (2, 4)
  *   "timing"       1 records HIP events around each phase (see nsp_stats_t) */
 int nsp_set_param(nsp_handle_t h, const char *name, int64_t value);
 

@@ -162,6 +162,16 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
   } else if (!std::strcmp(name, "sym_factor")) {
     if (value != 1 && value != 2) return nsp::set_error(h, NSP_ERR_INVALID, "sym_factor must be 1 or 2");
     h->sym_factor = (int)value;
+  } else if (!std::strcmp(name, "sym_rows_per_wave")) {
+    if (value != 1 && value != 2 && value != 4)
+      return nsp::set_error(h, NSP_ERR_INVALID, "sym_rows_per_wave must be 1, 2 or 4");
+    h->sym_rows_per_wave = (int)value;
+  } else if (!std::strcmp(name, "num_rows_per_wave")) {
+    if (value != 1 && value != 2) return nsp::set_error(h, NSP_ERR_INVALID, "num_rows_per_wave must be 1 or 2");
+    h->num_rows_per_wave = (int)value;
+  } else if (!std::strcmp(name, "num_table_factor")) {
+    if (value != 2 && value != 4) return nsp::set_error(h, NSP_ERR_INVALID, "num_table_factor must be 2 or 4");
+    h->num_table_factor = (int)value;
   } else if (!std::strcmp(name, "timing")) {
     h->timing = value != 0;
   } else if (!std::strcmp(name, "grid_cap")) {

@@ -53,6 +53,9 @@ struct Handle {
   int sub_lanes = 0;
   int sym_factor = 1;
   int grid_cap = 8;
+  int sym_rows_per_wave = 2;
+  int num_rows_per_wave = 2;
+  int num_table_factor = 2;
   bool timing = false;
   bool timing_pending = false;
   hipEvent_t events[8] = {};
@@ -69,7 +72,9 @@ struct Handle {
   const void *plan_a = nullptr, *plan_b = nullptr;
   int plan_sub = 16;
   int64_t num_count[kBins] = {0};
-  int64_t *a_rpt64 = nullptr, *b_rpt64 = nullptr, *c_rpt64 = nullptr, *intprod = nullptr;
+  int64_t *a_rpt64 = nullptr, *c_rpt64 = nullptr, *intprod = nullptr;
+  int64_t *ab_b0 = nullptr;   // per A entry: start of the B row it selects
+  int32_t *ab_len = nullptr;  // per A entry: length of that B row
   int32_t *row_nz = nullptr, *perm_sym = nullptr, *perm_num = nullptr;
   uint8_t *cls = nullptr;
   uint64_t *tile_status = nullptr;

@@ -53,14 +53,17 @@ __global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles) {
 
 // ---------------------------------------------------------------------------
 // Row info: intermediate products per row (nsp.cu:306-320), symbolic bin,
-// per-bin histogram, nprod, and int64 copies of A's and B's row pointers.
-// 16 lanes per row so the A.col reads are coalesced.
+// per-bin histogram, nprod, an int64 copy of A's row pointer, and for every
+// A entry (i,k) the start and length of B row k (ab_b0, ab_len), which the
+// hash passes then read with one independent load instead of the dependent
+// A.col -> B.row_ptr gather.  16 lanes per row so the A.col reads coalesce.
 template <typename RowT>
 __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
                                                  const int32_t *__restrict__ acol,
-                                                 const RowT *__restrict__ brpt, int64_t MB,
+                                                 const RowT *__restrict__ brpt,
                                                  int64_t *__restrict__ arpt64,
-                                                 int64_t *__restrict__ brpt64,
+                                                 int64_t *__restrict__ ab_b0,
+                                                 int32_t *__restrict__ ab_len,
                                                  int64_t *__restrict__ intprod,
                                                  int32_t *__restrict__ row_nz,
                                                  uint8_t *__restrict__ cls, Info *info) {
@@ -78,7 +81,11 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
     int64_t s = 0;
     for (int64_t p = p0 + glane; p < p1; p += GS) {
       const int32_t k = acol[p];
-      s += (int64_t)brpt[k + 1] - (int64_t)brpt[k];
+      const int64_t b0 = (int64_t)brpt[k];
+      const int64_t len = (int64_t)brpt[k + 1] - b0;
+      ab_b0[p] = b0;
+      ab_len[p] = (int32_t)len;
+      s += len;
     }
     s = group_sum<GS>(s);
     if (glane == 0) {
@@ -96,10 +103,7 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
       my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
     }
   }
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  for (int64_t j = t0; j <= MB; j += stride) brpt64[j] = (int64_t)brpt[j];
-  if (t0 == 0) arpt64[MA] = (int64_t)arpt[MA];
+  if (blockIdx.x == 0 && threadIdx.x == 0) arpt64[MA] = (int64_t)arpt[MA];
   my_sum = group_sum<64>(my_sum);
   my_max = group_max<64>(my_max);
   if ((threadIdx.x & 63) == 0) {
@@ -207,15 +211,15 @@ __global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64
                                                    const Info *__restrict__ info,
                                                    const int32_t *__restrict__ perm,
                                                    const int64_t *__restrict__ arpt,
-                                                   const int32_t *__restrict__ acol,
-                                                   const int64_t *__restrict__ brpt,
+                                                   const int64_t *__restrict__ ab_b0,
+                                                   const int32_t *__restrict__ ab_len,
                                                    const int32_t *__restrict__ bcol,
                                                    int32_t *__restrict__ row_nz) {
   constexpr int RPB = 256 / GS;
   constexpr int LOG = Log2<SH>::value;
   constexpr int SHD = SH + (GS < 16 ? 16 : GS);  // + one dummy slot per lane
   __shared__ int32_t keys[RPB][SHD];
-  __shared__ EntryScratch<GS, R, float> ent[RPB];
+  __shared__ EntryScratch<GS, float> ent[RPB];
   int64_t count, off;
   bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64
       wave_lds_sync();
       int cnt = 0;
       row_products<GS, R, false, float>(
-          glane, arpt[r], arpt[r + 1], 0, GS, acol, nullptr, brpt, bcol, nullptr, &ent[slot],
+          glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[slot],
           [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R]) {
             cnt += hash_insert_keys<LOG, R>(K, key, ok, glane);
           });
@@ -246,14 +250,14 @@ __global__ __launch_bounds__(BS) void k_sym_block(int bin_lo, int bin_hi, int64_
                                                   const Info *__restrict__ info,
                                                   const int32_t *__restrict__ perm,
                                                   const int64_t *__restrict__ arpt,
-                                                  const int32_t *__restrict__ acol,
-                                                  const int64_t *__restrict__ brpt,
+                                                  const int64_t *__restrict__ ab_b0,
+                                                  const int32_t *__restrict__ ab_len,
                                                   const int32_t *__restrict__ bcol,
                                                   int32_t *__restrict__ row_nz) {
   constexpr int LOG = Log2<SH>::value;
   constexpr int NW = BS / 64;
   __shared__ int32_t K[SH + 64];
-  __shared__ EntryScratch<64, R, float> ent[NW];
+  __shared__ EntryScratch<64, float> ent[NW];
   __shared__ int red[NW];
   int64_t count, off;
   bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
@@ -264,7 +268,7 @@ __global__ __launch_bounds__(BS) void k_sym_block(int bin_lo, int bin_hi, int64_
     __syncthreads();
     int cnt = 0;
     row_products<64, R, false, float>(
-        lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, acol, nullptr, brpt, bcol, nullptr, &ent[w],
+        lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[w],
         [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R]) {
           cnt += hash_insert_keys<LOG, R>(K, key, ok, lane);
         });
@@ -357,9 +361,9 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
                                                    const Info *__restrict__ info,
                                                    const int32_t *__restrict__ perm,
                                                    const int64_t *__restrict__ arpt,
-                                                   const int32_t *__restrict__ acol,
+                                                   const int64_t *__restrict__ ab_b0,
                                                    const V *__restrict__ aval,
-                                                   const int64_t *__restrict__ brpt,
+                                                   const int32_t *__restrict__ ab_len,
                                                    const int32_t *__restrict__ bcol,
                                                    const V *__restrict__ bval,
                                                    const int64_t *__restrict__ crpt,
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
   __shared__ int32_t keys[RPB][SHD];
   __shared__ V vals[RPB][SHD];
   __shared__ uint32_t stage[RPB][NS];
-  __shared__ EntryScratch<GS, R, V> ent[RPB];
+  __shared__ EntryScratch<GS, V> ent[RPB];
   int64_t count, off;
   bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
@@ -389,7 +393,7 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
       clear_lds<GS, SHD>(K, glane, kEmpty);
       clear_lds<GS, SHD>(W, glane, 0);
       wave_lds_sync();
-      row_products<GS, R, true, V>(glane, arpt[r], arpt[r + 1], 0, GS, acol, aval, brpt, bcol, bval,
+      row_products<GS, R, true, V>(glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, aval, bcol, bval,
                                    &ent[slot],
                                    [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
                                      hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, glane);
@@ -434,9 +438,9 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
                                                   const Info *__restrict__ info,
                                                   const int32_t *__restrict__ perm,
                                                   const int64_t *__restrict__ arpt,
-                                                  const int32_t *__restrict__ acol,
+                                                  const int64_t *__restrict__ ab_b0,
                                                   const V *__restrict__ aval,
-                                                  const int64_t *__restrict__ brpt,
+                                                  const int32_t *__restrict__ ab_len,
                                                   const int32_t *__restrict__ bcol,
                                                   const V *__restrict__ bval,
                                                   const int64_t *__restrict__ crpt,
@@ -448,7 +452,7 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
   __shared__ int32_t K[SH + 64];
   __shared__ V W[SH + 64];
   __shared__ uint32_t S[NS];
-  __shared__ EntryScratch<64, R, V> ent[NW];
+  __shared__ EntryScratch<64, V> ent[NW];
   __shared__ int scan_tmp[NW + 1];
   int64_t count, off;
   bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
@@ -458,7 +462,7 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
     clear_lds<BS, SH + 64>(K, threadIdx.x, kEmpty);
     clear_lds<BS, SH + 64>(W, threadIdx.x, 0);
     __syncthreads();
-    row_products<64, R, true, V>(lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, acol, aval, brpt, bcol, bval,
+    row_products<64, R, true, V>(lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, ab_b0, ab_len, aval, bcol, bval,
                                  &ent[w],
                                  [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
                                    hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, lane);
@@ -503,7 +507,7 @@ constexpr int kNumR = 4;  // B loads in flight per lane, numeric
 // Launches the kernel of bin `bin` over bins [lo, bin] (rows of lower bins fit
 // its larger table), or over all rows in natural order when direct >= 0.
 static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
-                           const int64_t *arpt, const int32_t *acol, const int64_t *brpt,
+                           const int64_t *arpt, const int64_t *ab_b0, const int32_t *ab_len,
                            const int32_t *bcol) {
   hipStream_t st = h->stream;
   const Info *info = h->d_info;
@@ -512,17 +516,26 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
   const int cap = h->grid_cap;
 #define SYM_GROUP(GS, SH)                                                                          \
   hipLaunchKernelGGL((k_sym_group<GS, SH, kSymR>), dim3(grid_for(h, count, 256 / GS, cap)),        \
-                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, acol, brpt, bcol, rnz)
+                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
 #define SYM_BLOCK(BS, SH)                                                                          \
   hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, cap)), dim3(BS), 0, \
-                     st, lo, bin, direct, info, perm, arpt, acol, brpt, bcol, rnz)
+                     st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
+  // bins 3..5: 1, 2 or 4 rows per wavefront (more rows in flight per CU
+  // against fewer lanes per row); tuned by "sym_rows_per_wave"
+  const int rpw = h->sym_rows_per_wave;
   switch (bin) {
     case 0: SYM_GROUP(16, 32); break;
     case 1: SYM_GROUP(32, 64); break;
     case 2: SYM_GROUP(64, 128); break;
-    case 3: SYM_GROUP(64, 256); break;
-    case 4: SYM_GROUP(64, 512); break;
-    case 5: SYM_GROUP(64, 1024); break;
+    case 3:
+      if (rpw >= 4) SYM_GROUP(16, 256); else if (rpw == 2) SYM_GROUP(32, 256); else SYM_GROUP(64, 256);
+      break;
+    case 4:
+      if (rpw >= 4) SYM_GROUP(16, 512); else if (rpw == 2) SYM_GROUP(32, 512); else SYM_GROUP(64, 512);
+      break;
+    case 5:
+      if (rpw >= 4) SYM_GROUP(16, 1024); else if (rpw == 2) SYM_GROUP(32, 1024); else SYM_GROUP(64, 1024);
+      break;
     case 6: SYM_BLOCK(256, 2048); break;
     case 7: SYM_BLOCK(512, 16384); break;
     default: break;
@@ -533,8 +546,8 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
 
 template <typename V>
 static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
-                           const int64_t *arpt, const int32_t *acol, const V *aval,
-                           const int64_t *brpt, const int32_t *bcol, const V *bval,
+                           const int64_t *arpt, const int64_t *ab_b0, const V *aval,
+                           const int32_t *ab_len, const int32_t *bcol, const V *bval,
                            int32_t *ccol, V *cval) {
   hipStream_t st = h->stream;
   const Info *info = h->d_info;
@@ -543,20 +556,27 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
   const int cap = h->grid_cap;
 #define NUM_GROUP(GS, NS, SHF)                                                                      \
   hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR>), dim3(grid_for(h, count, 256 / GS, cap)),      \
-                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, acol, aval, brpt, bcol,   \
-                     bval, crpt, ccol, cval)
+                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,      \
+                     bcol, bval, crpt, ccol, cval)
 #define NUM_BLOCK(BS, NS, SHF)                                                                      \
   hipLaunchKernelGGL((k_num_block<V, BS, NS, SHF, kNumR>), dim3(grid_for(h, count, 1, cap)), dim3(BS),   \
-                     0, st, lo, bin, direct, info, perm, arpt, acol, aval, brpt, bcol, bval, crpt,  \
-                     ccol, cval)
+                     0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval,     \
+                     crpt, ccol, cval)
+  // bins 3..4: 1 or 2 rows per wavefront ("num_rows_per_wave"); table
+  // factor 2 or 4 slots per row entry ("num_table_factor") for bins 0..4
+  const int rpw = h->num_rows_per_wave;
+  const bool f4 = h->num_table_factor >= 4;
   switch (bin) {
-    // table factor 4 for the small bins (near-collision-free probes), 2 where
-    // the LDS footprint would limit occupancy
-    case 0: NUM_GROUP(16, 16, 4); break;
-    case 1: NUM_GROUP(32, 32, 4); break;
-    case 2: NUM_GROUP(64, 64, 4); break;
-    case 3: NUM_GROUP(64, 128, 4); break;
-    case 4: NUM_GROUP(64, 256, 2); break;
+    case 0: if (f4) NUM_GROUP(16, 16, 4); else NUM_GROUP(16, 16, 2); break;
+    case 1: if (f4) NUM_GROUP(32, 32, 4); else NUM_GROUP(32, 32, 2); break;
+    case 2: if (f4) NUM_GROUP(64, 64, 4); else NUM_GROUP(64, 64, 2); break;
+    case 3:
+      if (rpw >= 2) { if (f4) NUM_GROUP(32, 128, 4); else NUM_GROUP(32, 128, 2); }
+      else { if (f4) NUM_GROUP(64, 128, 4); else NUM_GROUP(64, 128, 2); }
+      break;
+    case 4:
+      if (rpw >= 2) NUM_GROUP(32, 256, 2); else NUM_GROUP(64, 256, 2);
+      break;
     case 5: NUM_GROUP(64, 512, 2); break;
     case 6: NUM_BLOCK(256, 2048, 2); break;
     case 7: NUM_BLOCK(512, 4096, 2); break;
@@ -636,10 +656,11 @@ int collect_timing(Handle *h) {
 }
 
 // ---------------------------------------------------------------------------
-static int ensure_workspace(Handle *h, int64_t MA, int64_t MB) {
+static int ensure_workspace(Handle *h, int64_t MA, int64_t nnzA) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = MA / kScanTile + 2;
-  size_t need = al((MA + 1) * 8) * 3 + al((MB + 1) * 8) + al(MA * 4) * 3 + al(MA + 1) + al(ntiles * 8);
+  size_t need = al((MA + 1) * 8) * 3 + al(nnzA * 8 + 8) + al(nnzA * 4 + 4) + al(MA * 4) * 3 + al(MA + 1) +
+                al(ntiles * 8);
   if (need > h->ws.bytes) {
     if (h->ws.base) NSP_HIP(h, hipFree(h->ws.base));
     h->ws.base = nullptr;
@@ -656,7 +677,8 @@ static int ensure_workspace(Handle *h, int64_t MA, int64_t MB) {
   h->a_rpt64 = (int64_t *)take((MA + 1) * 8);
   h->c_rpt64 = (int64_t *)take((MA + 1) * 8);
   h->intprod = (int64_t *)take((MA + 1) * 8);
-  h->b_rpt64 = (int64_t *)take((MB + 1) * 8);
+  h->ab_b0 = (int64_t *)take(nnzA * 8 + 8);
+  h->ab_len = (int32_t *)take(nnzA * 4 + 4);
   h->row_nz = (int32_t *)take(MA * 4);
   h->perm_sym = (int32_t *)take(MA * 4);
   h->perm_num = (int32_t *)take(MA * 4);
@@ -689,15 +711,14 @@ static int read_info(Handle *h) {
 // Row info + symbolic binning (no host sync).  Also used by nsp_intprod.
 template <typename RowT>
 static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B) {
-  const int64_t MA = A->rows, MB = B->rows;
-  NSP_TRY(ensure_workspace(h, MA, MB));
+  const int64_t MA = A->rows;
+  NSP_TRY(ensure_workspace(h, MA, A->nnz));
   timing_mark(h, 0);
   hipLaunchKernelGGL(k_init, dim3(std::max<int64_t>(1, std::min<int64_t>((h->ntiles + 255) / 256, 1024))),
                      dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles);
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(std::max((MA + 15) / 16, (MB + 256) / 256),
-                                                            (int64_t)h->num_cu * 8));
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 15) / 16, (int64_t)h->num_cu * 8));
   hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr,
-                     A->col, (const RowT *)B->row_ptr, MB, h->a_rpt64, h->b_rpt64, h->intprod,
+                     A->col, (const RowT *)B->row_ptr, h->a_rpt64, h->ab_b0, h->ab_len, h->intprod,
                      h->row_nz, h->cls, h->d_info);
   NSP_HIP(h, hipGetLastError());
   return NSP_OK;
@@ -723,7 +744,7 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
       launch_scatter(h, MA, h->d_info->count_sym, h->d_info->cursor_sym, h->perm_sym);
     timing_mark(h, 1);
     for (int q = 0; q < n; q++)
-      launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, A->col, h->b_rpt64, B->col);
+      launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, h->ab_len, B->col);
     h->stats.sym_launches = n;
   }
   timing_mark(h, 2);
@@ -789,8 +810,8 @@ static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
     launch_scatter(h, A->rows, h->d_info->count_num, h->d_info->cursor_num, h->perm_num);
   timing_mark(h, 4);
   for (int q = 0; q < n; q++)
-    launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, A->col, (const V *)A->val,
-                      h->b_rpt64, B->col, (const V *)B->val, C->col, (V *)C->val);
+    launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, (const V *)A->val,
+                      h->ab_len, B->col, (const V *)B->val, C->col, (V *)C->val);
   h->stats.num_launches = n;
   timing_mark(h, 5);
   h->timing_pending = h->timing;
