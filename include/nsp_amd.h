@@ -144,6 +144,57 @@ int nsp_intprod(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *A, const nsp_cs
  * src/CSR.cu:1485-1513.) */
 int nsp_check_csr(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row);
 
+/* ---- batched products and graph capture --------------------------------- */
+/* Symbolic phase of `count` independent products C_b = A_b * B_b stored as
+ * concatenated CSR: A holds the rows of A_0, A_1, ... one after another
+ * (A->row_ptr runs over all of them), with columns local to each matrix;
+ * a_row_off[b] is A_b's first row (device int64, count+1 entries, ending at
+ * A->rows); B alike.  C comes out concatenated the same way (rows of C_b at
+ * a_row_off[b]..).  Follow with nsp_spgemm_numeric on the same handle.  No
+ * reference counterpart: the reference can only loop nsp_spgemm_kernel_hash
+ * (nsp/src/nsp.cu:2448) over the matrices. */
+int nsp_spgemm_batched_symbolic(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A,
+                                const int64_t *a_row_off, const nsp_csr_t *B, const int64_t *b_row_off,
+                                int64_t count, void *c_row_ptr, int64_t *c_nnz);
+
+typedef struct nsp_graph_s *nsp_graph_t;
+
+/* Records the whole product -- binning, symbolic pass, scan, numeric pass --
+ * into one hipGraph with no host synchronisation inside, for repeated
+ * launches over the same A and B buffers (values may change between
+ * launches).  a_row_off == NULL records a single product; otherwise a batch
+ * as in nsp_spgemm_batched_symbolic.  C->row_ptr (A->rows+1 entries) and
+ * C->col / C->val (capacity C->nnz entries) are written by every launch;
+ * rows that would pass the capacity are skipped and reported by
+ * nsp_graph_result.  The graph launches on the handle's stream at creation.
+ * No reference counterpart (the reference synchronises the host 5-9 times
+ * per product, nsp/src/nsp.cu:417-471, 921-1130, 2178-2440). */
+int nsp_graph_create(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A,
+                     const int64_t *a_row_off, const nsp_csr_t *B, const int64_t *b_row_off,
+                     int64_t count, nsp_csr_t *C, nsp_graph_t *g);
+int nsp_graph_launch(nsp_graph_t g);                  /* asynchronous */
+int nsp_graph_result(nsp_graph_t g, int64_t *nnz);    /* synchronises; nnz(C) and status */
+const char *nsp_graph_last_error(nsp_graph_t g);
+int nsp_graph_destroy(nsp_graph_t g);
+
+/* ---- backward pair on fixed patterns ------------------------------------ */
+/* Transpose of X into XT (XT->rows = X->cols, row pointer, columns and values
+ * caller-allocated; XT rows come out sorted by X's row index).  Replaces the
+ * reference's cuSPARSE csr2csc transpose CSRm::T (src/CSR.cu:739-797); cached
+ * by the caller as A^T for nsp_spgemm_backward.  Columns of X with more than
+ * 4096 entries return NSP_ERR_UNSUPPORTED. */
+int nsp_transpose(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp_csr_t *XT);
+
+/* Gradients of C = A*B for a loss L(C) on fixed patterns, given dC = dL/dC on
+ * C's pattern (C->row_ptr, C->col from the forward; dC aligned with C->col):
+ *   dA = (dC B^T) sampled on pat(A)   (dA aligned with A->col)
+ *   dB = (A^T dC) sampled on pat(B)   (dB aligned with B->col)
+ * AT is A's transpose from nsp_transpose.  Deterministic.  No reference
+ * counterpart (the reference has no backward); C rows of up to 1024 entries. */
+int nsp_spgemm_backward(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A,
+                        const nsp_csr_t *B, const nsp_csr_t *AT, const nsp_csr_t *C,
+                        const void *dC, void *dA, void *dB);
+
 /* ---- one rank's local share on one device ------------------------------- */
 /* Column compaction ("shrink"): perm = sorted unique(col U [0..l]),
  * new_col[e] = position of col[e] in perm, *shrinked_m = |perm|.  Every

@@ -157,6 +157,50 @@ class Handle:
         self._check(self.L.nsp_check_csr(self.h, X.index_type, C.byref(x), C.byref(bad)))
         return int(bad.value)
 
+    # -- batched products and graphs ------------------------------------------
+    def batched_symbolic(self, A: DeviceCSR, a_off, B: DeviceCSR, b_off):
+        """Symbolic phase of the batch (concatenated rows, int64 device row
+        offsets); returns (C row_ptr, nnz); follow with numeric()."""
+        torch = _torch()
+        _same_types(A, B)
+        rpt = torch.empty(A.nrows + 1, dtype=A.row_ptr.dtype, device=A.row_ptr.device)
+        nnz = C.c_int64()
+        a, b = A.c_struct(), B.c_struct()
+        self._check(self.L.nsp_spgemm_batched_symbolic(
+            self.h, A.index_type, A.value_type, C.byref(a), C.c_void_p(a_off.data_ptr()), C.byref(b),
+            C.c_void_p(b_off.data_ptr()), a_off.numel() - 1, C.c_void_p(rpt.data_ptr()), C.byref(nnz)))
+        return rpt, int(nnz.value)
+
+    def batched_spgemm(self, A: DeviceCSR, a_off, B: DeviceCSR, b_off) -> DeviceCSR:
+        rpt, nnz = self.batched_symbolic(A, a_off, B, b_off)
+        return self.numeric(A, B, rpt, nnz)
+
+    def graph(self, A: DeviceCSR, B: DeviceCSR, capacity: int, a_off=None, b_off=None) -> "Graph":
+        return Graph(self, A, B, capacity, a_off, b_off)
+
+    # -- backward pair -----------------------------------------------------
+    def transpose(self, X: DeviceCSR) -> DeviceCSR:
+        torch = _torch()
+        XT = DeviceCSR(torch.empty(X.ncols + 1, dtype=X.row_ptr.dtype, device=X.col.device),
+                       torch.empty(X.nnz, dtype=torch.int32, device=X.col.device),
+                       torch.empty(X.nnz, dtype=X.val.dtype, device=X.col.device), X.ncols, X.nrows)
+        x, xt = X.c_struct(), XT.c_struct()
+        self._check(self.L.nsp_transpose(self.h, X.index_type, X.value_type, C.byref(x), C.byref(xt)))
+        return XT
+
+    def backward(self, A: DeviceCSR, B: DeviceCSR, AT: DeviceCSR, Cpat: DeviceCSR, dC):
+        """(dA, dB) aligned with A.col and B.col for dC aligned with Cpat.col."""
+        torch = _torch()
+        dA = torch.empty(A.nnz, dtype=A.val.dtype, device=A.col.device)
+        dB = torch.empty(B.nnz, dtype=B.val.dtype, device=B.col.device)
+        a, b, at, c = A.c_struct(), B.c_struct(), AT.c_struct(), Cpat.c_struct()
+        self._check(self.L.nsp_spgemm_backward(
+            self.h, A.index_type, A.value_type, C.byref(a), C.byref(b), C.byref(at), C.byref(c),
+            C.c_void_p(dC.data_ptr() if dC.numel() else None),
+            C.c_void_p(dA.data_ptr() if dA.numel() else None),
+            C.c_void_p(dB.data_ptr() if dB.numel() else None)))
+        return dA, dB
+
     # -- one rank's local share -------------------------------------------
     def shrink_cols(self, col, l: int, lo: int, hi: int):
         """(new_col, perm): perm = sorted unique(col U [0..l]) (int32 device
@@ -191,6 +235,57 @@ class Handle:
                                                 C.c_void_p(rows.data_ptr() if nr else None),
                                                 int(row_offset), int(col_shift), C.byref(o)))
         return out
+
+
+class Graph:
+    """A product (or batch of products) recorded into one hipGraph; launch()
+    re-runs binning, symbolic pass, scan and numeric pass with no host
+    synchronisation; result() synchronises and returns C (row_ptr, col and
+    val views of nnz entries)."""
+
+    def __init__(self, handle: Handle, A: DeviceCSR, B: DeviceCSR, capacity: int, a_off=None, b_off=None):
+        torch = _torch()
+        _same_types(A, B)
+        self.handle, self.L = handle, handle.L
+        self.keep = (A, B, a_off, b_off)
+        dev = A.col.device
+        self.C = DeviceCSR(torch.empty(A.nrows + 1, dtype=A.row_ptr.dtype, device=dev),
+                           torch.empty(max(capacity, 1), dtype=torch.int32, device=dev),
+                           torch.empty(max(capacity, 1), dtype=A.val.dtype, device=dev), A.nrows, B.ncols)
+        c = _lib.nsp_csr_t(A.nrows, B.ncols, capacity, self.C.row_ptr.data_ptr(), self.C.col.data_ptr(),
+                           self.C.val.data_ptr())
+        a, b = A.c_struct(), B.c_struct()
+        self.g = C.c_void_p()
+        count = a_off.numel() - 1 if a_off is not None else 0
+        handle._check(self.L.nsp_graph_create(
+            handle.h, A.index_type, A.value_type, C.byref(a),
+            C.c_void_p(a_off.data_ptr() if a_off is not None else None), C.byref(b),
+            C.c_void_p(b_off.data_ptr() if b_off is not None else None), count, C.byref(c),
+            C.byref(self.g)))
+
+    def _check(self, s):
+        if s != _lib.NSP_OK:
+            raise NspError(s, self.L.nsp_graph_last_error(self.g).decode())
+
+    def launch(self):
+        self._check(self.L.nsp_graph_launch(self.g))
+
+    def result(self) -> DeviceCSR:
+        nnz = C.c_int64()
+        self._check(self.L.nsp_graph_result(self.g, C.byref(nnz)))
+        n = int(nnz.value)
+        return DeviceCSR(self.C.row_ptr, self.C.col[:n], self.C.val[:n], self.C.nrows, self.C.ncols)
+
+    def close(self):
+        if getattr(self, "g", None):
+            self.L.nsp_graph_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _same_types(A: DeviceCSR, B: DeviceCSR):

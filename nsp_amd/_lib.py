@@ -60,6 +60,14 @@ SIGNATURES = [
     ("nsp_shrink_cols", _I, [_H, _I64, _VP, _I64, _I64, _I64, _VP, _VP, _I64, _P64]),
     ("nsp_gather_rows_count", _I, [_H, _I, _CSRP, _I64, _VP, _I64, _VP, _P64]),
     ("nsp_gather_rows_fill", _I, [_H, _I, _I, _CSRP, _I64, _VP, _I64, _I64, _CSRP]),
+    ("nsp_spgemm_batched_symbolic", _I, [_H, _I, _I, _CSRP, _VP, _CSRP, _VP, _I64, _VP, _P64]),
+    ("nsp_graph_create", _I, [_H, _I, _I, _CSRP, _VP, _CSRP, _VP, _I64, _CSRP, C.POINTER(_H)]),
+    ("nsp_graph_launch", _I, [_H]),
+    ("nsp_graph_result", _I, [_H, _P64]),
+    ("nsp_graph_last_error", C.c_char_p, [_H]),
+    ("nsp_graph_destroy", _I, [_H]),
+    ("nsp_transpose", _I, [_H, _I, _I, _CSRP, _CSRP]),
+    ("nsp_spgemm_backward", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _CSRP, _VP, _VP, _VP]),
 ]
 
 

@@ -1,0 +1,339 @@
+// Backward pair of C = A*B on fixed sparsity patterns, and the CSR transpose
+// it caches.  Given dC = dL/dC on C's pattern:
+//   dA = (dC * B^T) sampled on pat(A):  dA_ik = sum_{j in B_k} B_kj dC_ij
+//   dB = (A^T * dC) sampled on pat(B):  dB_kj = sum_{i in A^T_k} A_ik dC_ij
+// Every (i, j) a forward product reaches lies in pat(C), so both are sums over
+// the forward's intermediate products; no new structure is created.
+//
+// dA walks A's rows with the forward enumerator (row_products.hpp): C row i
+// goes into an LDS hash (column -> dC_ij), and each product of entry (i,k)
+// adds B_kj * dC_ij into that entry's LDS accumulator.  dB walks B's rows with
+// the cached A^T (rows sorted by i, so the sum order is fixed): for every
+// (i, A_ik) of A^T row k and every j in B_k it binary-searches dC_ij in C
+// row i.  Both are deterministic.
+//
+// The reference has no backward; its only transpose is cuSPARSE csr2csc
+// (src/CSR.cu:739-797).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "device_utils.hpp"
+#include "nsp_internal.hpp"
+#include "row_products.hpp"
+#include "scan.hpp"
+#include "spgemm_internal.hpp"
+
+namespace nsp {
+
+// ---------------------------------------------------------------------------
+// transpose: count per column -> scan -> scatter -> sort each row of X^T by
+// its (original row) index so that X^T is a sorted CSR, as csr2csc gives it.
+__global__ void k_t_zero(int64_t n, int64_t *cnt, uint64_t *status, int64_t ntiles, uint32_t *ticket) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t i = t; i < n; i += (int64_t)gridDim.x * blockDim.x) cnt[i] = 0;
+  for (int64_t i = t; i < ntiles; i += (int64_t)gridDim.x * blockDim.x) status[i] = 0;
+  if (t == 0) ticket[0] = 0;
+}
+
+template <typename RowT>
+__global__ void k_t_count(int64_t nnz, const int32_t *__restrict__ col, int64_t *__restrict__ cnt) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd((unsigned long long *)&cnt[col[e]], 1ull);
+}
+
+template <typename RowT, typename V>
+__global__ void k_t_scatter(int64_t rows, const RowT *__restrict__ rpt, const int32_t *__restrict__ col,
+                            const V *__restrict__ val, const RowT *__restrict__ trpt,
+                            int64_t *__restrict__ cursor, int32_t *__restrict__ tcol, V *__restrict__ tval) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t p = (int64_t)rpt[i]; p < (int64_t)rpt[i + 1]; p++) {
+      const int32_t c = col[p];
+      const int64_t d = (int64_t)trpt[c] + (int64_t)atomicAdd((unsigned long long *)&cursor[c], 1ull);
+      tcol[d] = (int32_t)i;
+      tval[d] = val[p];
+    }
+  }
+}
+
+// One block of 256 threads per X^T row: bitonic sort of (row index, value)
+// pairs in LDS, rows up to kTSortMax entries.
+constexpr int kTSortMax = 4096;
+
+template <typename RowT, typename V>
+__global__ __launch_bounds__(256) void k_t_sort(int64_t trows, const RowT *__restrict__ trpt,
+                                                int32_t *__restrict__ tcol, V *__restrict__ tval,
+                                                uint32_t *__restrict__ too_long) {
+  __shared__ uint64_t S[kTSortMax];  // (row index << 32) | position in the row
+  __shared__ V Vt[kTSortMax];
+  for (int64_t r = blockIdx.x; r < trows; r += gridDim.x) {
+    const int64_t p0 = (int64_t)trpt[r], n = (int64_t)trpt[r + 1] - p0;
+    if (n <= 1) continue;
+    if (n > kTSortMax) {
+      if (threadIdx.x == 0) *too_long = 1;
+      continue;
+    }
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (int g = threadIdx.x; g < n2; g += 256) {
+      if (g < n) {
+        S[g] = ((uint64_t)(uint32_t)tcol[p0 + g] << 32) | (uint32_t)g;
+        Vt[g] = tval[p0 + g];
+      } else {
+        S[g] = ~0ull;
+      }
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int t = threadIdx.x; t < n2 / 2; t += 256) {
+          const int i = ((t / j) * 2 * j) + (t % j);
+          const int q = i + j;
+          const bool asc = (i & k) == 0;
+          const uint64_t x = S[i], y = S[q];
+          if (asc ? (x > y) : (x < y)) {
+            S[i] = y;
+            S[q] = x;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int g = threadIdx.x; g < n; g += 256) {
+      tcol[p0 + g] = (int32_t)(S[g] >> 32);
+      tval[p0 + g] = Vt[(uint32_t)S[g]];
+    }
+    __syncthreads();
+  }
+}
+
+static int aux2_reserve(Handle *h, size_t bytes) {
+  if (bytes <= h->aux2.bytes) return NSP_OK;
+  if (h->aux2.base) NSP_HIP(h, hipFree(h->aux2.base));
+  h->aux2.base = nullptr;
+  h->aux2.bytes = 0;
+  if (hipMalloc(&h->aux2.base, bytes + bytes / 4) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(h, NSP_ERR_OOM, "transpose workspace allocation failed");
+  }
+  h->aux2.bytes = bytes + bytes / 4;
+  return NSP_OK;
+}
+
+template <typename RowT, typename V>
+static int transpose_impl(Handle *h, const nsp_csr_t *X, nsp_csr_t *XT) {
+  const int64_t n = X->cols, nnz = X->nnz;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const int64_t ntiles = n / kScanTile + 2;
+  NSP_TRY(aux2_reserve(h, al((n + 1) * 8) + al(ntiles * 8) + 256));
+  char *p = (char *)h->aux2.base;
+  int64_t *cnt = (int64_t *)p;
+  uint64_t *status = (uint64_t *)(p + al((n + 1) * 8));
+  uint32_t *ticket = (uint32_t *)(p + al((n + 1) * 8) + al(ntiles * 8));
+  uint32_t *too_long = ticket + 1;
+  hipStream_t st = h->stream;
+  const int g =(int)std::max<int64_t>(1, std::min<int64_t>((std::max(n, nnz) + 255) / 256, (int64_t)h->num_cu * 16));
+  hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, status, ntiles, ticket);
+  hipLaunchKernelGGL((k_t_count<RowT>), dim3(g), dim3(256), 0, st, nnz, X->col, cnt);
+  hipLaunchKernelGGL((k_exscan<int64_t, RowT>), dim3((unsigned)(n / kScanTile + 1)), dim3(kScanBS), 0, st, n, cnt,
+                     (RowT *)XT->row_ptr, status, ticket);
+  // reuse cnt as the scatter cursor
+  hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, status, 0, too_long);
+  hipLaunchKernelGGL((k_t_scatter<RowT, V>), dim3(g), dim3(256), 0, st, X->rows, (const RowT *)X->row_ptr, X->col,
+                     (const V *)X->val, (const RowT *)XT->row_ptr, cnt, XT->col, (V *)XT->val);
+  hipLaunchKernelGGL((k_t_sort<RowT, V>), dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)h->num_cu * 8))),
+                     dim3(256), 0, st, n, (const RowT *)XT->row_ptr, XT->col, (V *)XT->val, too_long);
+  NSP_HIP(h, hipGetLastError());
+  uint32_t tl = 0;
+  NSP_HIP(h, hipMemcpyAsync(&tl, too_long, 4, hipMemcpyDeviceToHost, st));
+  NSP_HIP(h, hipStreamSynchronize(st));
+  if (tl) return set_error(h, NSP_ERR_UNSUPPORTED, "transpose: a column holds more than 4096 entries");
+  return NSP_OK;
+}
+
+int transpose(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp_csr_t *XT) {
+  if (!X || !XT || !X->row_ptr || !XT->row_ptr || X->nnz != XT->nnz || XT->rows != X->cols ||
+      (X->nnz > 0 && (!X->col || !X->val || !XT->col || !XT->val)))
+    return set_error(h, NSP_ERR_INVALID, "nsp_transpose: bad arguments");
+  if (it == NSP_IDX32 && vt == NSP_F64) return transpose_impl<int32_t, double>(h, X, XT);
+  if (it == NSP_IDX32) return transpose_impl<int32_t, float>(h, X, XT);
+  if (vt == NSP_F64) return transpose_impl<int64_t, double>(h, X, XT);
+  return transpose_impl<int64_t, float>(h, X, XT);
+}
+
+// ---------------------------------------------------------------------------
+// dA: GS lanes per A row; C row i (nz <= NS) in an LDS table column -> dC.
+template <typename V, typename RowT, int GS, int NS, int R>
+__global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const RowT *__restrict__ arpt,
+                                                const int64_t *__restrict__ ab_b0,
+                                                const int32_t *__restrict__ ab_len,
+                                                const int32_t *__restrict__ bcol, const V *__restrict__ bval,
+                                                const RowT *__restrict__ crpt, const int32_t *__restrict__ ccol,
+                                                const V *__restrict__ dc, V *__restrict__ da,
+                                                uint32_t *__restrict__ too_long) {
+  constexpr int RPB = 256 / GS;
+  constexpr int SH = 2 * NS;
+  constexpr int LOG = Log2<SH>::value;
+  __shared__ int32_t keys[RPB][SH];
+  __shared__ V vals[RPB][SH];
+  __shared__ V acc[RPB][GS];
+  __shared__ EntryScratch<GS, V> ent[RPB];
+  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
+  int32_t *K = keys[slot];
+  V *W = vals[slot];
+  for (int64_t i = blockIdx.x * (int64_t)RPB + slot; i < MA; i += (int64_t)gridDim.x * RPB) {
+    const int64_t c0 = (int64_t)crpt[i], nz = (int64_t)crpt[i + 1] - c0;
+    if (nz > NS) {
+      if (glane == 0) *too_long = 1;
+      continue;
+    }
+    for (int s = glane; s < SH; s += GS) K[s] = kEmpty;
+    wave_lds_sync();
+    for (int64_t q = glane; q < nz; q += GS) {
+      const int32_t key = ccol[c0 + q];
+      uint32_t hs = hash_slot<LOG>(key);
+      while (atomicCAS(&K[hs], kEmpty, key) != kEmpty) hs = (hs + 1) & (SH - 1);
+      W[hs] = dc[c0 + q];
+    }
+    wave_lds_sync();
+    const int64_t a0 = (int64_t)arpt[i], a1 = (int64_t)arpt[i + 1];
+    for (int64_t cc = a0; cc < a1; cc += GS) {
+      // one chunk of GS entries at a time so acc[] maps to the chunk
+      acc[slot][glane] = (V)0;
+      wave_lds_sync();
+      row_products<GS, R, false, V, true>(glane, cc, (cc + GS < a1 ? cc + GS : a1), 0, GS, ab_b0, ab_len, nullptr,
+                                    bcol, bval, &ent[slot],
+                                    [&](const int32_t(&key)[R], const V(&)[R], const bool(&ok)[R],
+                                        const V(&bv)[R], const int(&e)[R]) {
+#pragma unroll
+                                      for (int u = 0; u < R; u++)
+                                        if (ok[u])
+                                          __hip_atomic_fetch_add(&acc[slot][e[u]],
+                                                                 bv[u] * hash_lookup<LOG, V>(K, W, key[u]),
+                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                    });
+      wave_lds_sync();
+      // map compacted entry index back to the entry's position
+      const int64_t ja = cc + glane;
+      int len = 0;
+      if (ja < a1) len = ab_len[ja];
+      const uint64_t nzm = group_ballot<GS>(len > 0);
+      const int cidx = __popcll(nzm & ((1ull << glane) - 1));
+      if (ja < a1) da[ja] = len > 0 ? acc[slot][cidx] : (V)0;
+      wave_lds_sync();
+    }
+  }
+}
+
+// dB: one GS-lane group per B row k; lanes walk the pairs (t, q) of A^T row
+// k's entries t and B row k's entries q; dC_ij found by binary search in the
+// sorted C row i.
+template <typename V, typename RowT, int GS>
+__global__ __launch_bounds__(256) void k_grad_b(int64_t MB, const RowT *__restrict__ brpt,
+                                                const int32_t *__restrict__ bcol, const RowT *__restrict__ atrpt,
+                                                const int32_t *__restrict__ atcol, const V *__restrict__ atval,
+                                                const RowT *__restrict__ crpt, const int32_t *__restrict__ ccol,
+                                                const V *__restrict__ dc, V *__restrict__ db) {
+  constexpr int RPB = 256 / GS;
+  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
+  for (int64_t k = blockIdx.x * (int64_t)RPB + slot; k < MB; k += (int64_t)gridDim.x * RPB) {
+    const int64_t b0 = (int64_t)brpt[k], nb = (int64_t)brpt[k + 1] - b0;
+    const int64_t t0 = (int64_t)atrpt[k], nt = (int64_t)atrpt[k + 1] - t0;
+    // lanes take B entries q = glane, glane+GS, ...; each sums over all t in order
+    for (int64_t q = glane; q < nb; q += GS) {
+      const int32_t j = bcol[b0 + q];
+      V s = (V)0;
+      for (int64_t t = 0; t < nt; t++) {
+        const int32_t i = atcol[t0 + t];
+        const int64_t end = (int64_t)crpt[i + 1];
+        int64_t lo = (int64_t)crpt[i], hi = end;
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (ccol[mid] < j) lo = mid + 1; else hi = mid;
+        }
+        // j is in C row i whenever A_ik and B_kj are stored (C = A*B's pattern)
+        if (lo < end && ccol[lo] == j) s += atval[t0 + t] * dc[lo];
+      }
+      db[b0 + q] = s;
+    }
+  }
+}
+
+// Largest row length of a CSR (for the dA table size), into *out (zeroed).
+template <typename RowT>
+__global__ void k_max_rowlen(int64_t rows, const RowT *__restrict__ rpt, unsigned long long *out) {
+  unsigned long long m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long l = (unsigned long long)((int64_t)rpt[i + 1] - (int64_t)rpt[i]);
+    m = l > m ? l : m;
+  }
+  m = group_max<64>(m);
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ void k_zero_words(uint32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+template <typename V, typename RowT>
+static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const nsp_csr_t *AT,
+                         const nsp_csr_t *C, const V *dc, V *da, V *db) {
+  hipStream_t st = h->stream;
+  // per-entry B row extents: the binning pass of the forward
+  NSP_TRY(spgemm_row_extents(h, sizeof(RowT) == 4 ? NSP_IDX32 : NSP_IDX64, A, B));
+  NSP_TRY(aux2_reserve(h, 256));
+  uint32_t *flags = (uint32_t *)h->aux2.base;  // [0..1] max C row length, [2] too long
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 4);
+  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>((C->rows + 255) / 256, (int64_t)h->num_cu * 4));
+  hipLaunchKernelGGL((k_max_rowlen<RowT>), dim3(g1), dim3(256), 0, st, C->rows, (const RowT *)C->row_ptr,
+                     (unsigned long long *)flags);
+  unsigned long long maxnz = 0;
+  NSP_HIP(h, hipMemcpyAsync(&maxnz, flags, 8, hipMemcpyDeviceToHost, st));
+  NSP_HIP(h, hipStreamSynchronize(st));
+  uint32_t *too_long = flags + 2;
+  const int64_t MA = A->rows;
+  if (MA > 0) {
+    if (maxnz <= 128) {
+      const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 8));
+      hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128, 4>), dim3(ga), dim3(256), 0, st, MA, (const RowT *)A->row_ptr,
+                         h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
+                         too_long);
+    } else if (maxnz <= 1024) {
+      const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 3) / 4, (int64_t)h->num_cu * 4));
+      hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024, 4>), dim3(ga), dim3(256), 0, st, MA,
+                         (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col, (const V *)B->val,
+                         (const RowT *)C->row_ptr, C->col, dc, da, too_long);
+    } else {
+      return set_error(h, NSP_ERR_UNSUPPORTED, "backward: C rows with more than 1024 entries");
+    }
+  }
+  if (B->rows > 0) {
+    const int gb = (int)std::max<int64_t>(1, std::min<int64_t>((B->rows + 15) / 16, (int64_t)h->num_cu * 8));
+    hipLaunchKernelGGL((k_grad_b<V, RowT, 16>), dim3(gb), dim3(256), 0, st, B->rows, (const RowT *)B->row_ptr,
+                       B->col, (const RowT *)AT->row_ptr, AT->col, (const V *)AT->val, (const RowT *)C->row_ptr,
+                       C->col, dc, db);
+  }
+  NSP_HIP(h, hipGetLastError());
+  uint32_t tl = 0;
+  NSP_HIP(h, hipMemcpyAsync(&tl, too_long, 4, hipMemcpyDeviceToHost, st));
+  NSP_HIP(h, hipStreamSynchronize(st));
+  if (tl) return set_error(h, NSP_ERR_UNSUPPORTED, "backward: a C row exceeds the dA table");
+  return NSP_OK;
+}
+
+int spgemm_backward(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                    const nsp_csr_t *AT, const nsp_csr_t *C, const void *dc, void *da, void *db) {
+  if (!A || !B || !AT || !C || !A->row_ptr || !B->row_ptr || !AT->row_ptr || !C->row_ptr ||
+      A->cols != B->rows || AT->rows != A->cols || C->rows != A->rows)
+    return set_error(h, NSP_ERR_INVALID, "nsp_spgemm_backward: bad arguments");
+  if ((A->nnz && !da) || (B->nnz && !db) || (C->nnz && !dc))
+    return set_error(h, NSP_ERR_INVALID, "nsp_spgemm_backward: NULL gradient buffer");
+  if (it == NSP_IDX32 && vt == NSP_F64)
+    return backward_impl<double, int32_t>(h, A, B, AT, C, (const double *)dc, (double *)da, (double *)db);
+  if (it == NSP_IDX32) return backward_impl<float, int32_t>(h, A, B, AT, C, (const float *)dc, (float *)da, (float *)db);
+  if (vt == NSP_F64)
+    return backward_impl<double, int64_t>(h, A, B, AT, C, (const double *)dc, (double *)da, (double *)db);
+  return backward_impl<float, int64_t>(h, A, B, AT, C, (const float *)dc, (float *)da, (float *)db);
+}
+
+}  // namespace nsp

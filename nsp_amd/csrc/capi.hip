@@ -109,6 +109,7 @@ int nsp_destroy(nsp_handle_t hs) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->ws.base) (void)hipFree(h->ws.base);
   if (h->aux.base) (void)hipFree(h->aux.base);
+  if (h->aux2.base) (void)hipFree(h->aux2.base);
   for (hipEvent_t e : h->events)
     if (e) (void)hipEventDestroy(e);
   if (h->d_info) (void)hipFree(h->d_info);
@@ -280,6 +281,56 @@ int nsp_gather_rows_count(nsp_handle_t hs, nsp_index_t it, const nsp_csr_t *B, i
                           int64_t row_offset, void *out_row_ptr, int64_t *out_nnz) {
   if (!hs) return NSP_ERR_INVALID;
   return nsp::gather_rows_count(&hs->h, it, B, nr, rows, row_offset, out_row_ptr, out_nnz);
+}
+
+int nsp_spgemm_batched_symbolic(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A,
+                                const int64_t *a_row_off, const nsp_csr_t *B, const int64_t *b_row_off,
+                                int64_t count, void *c_row_ptr, int64_t *c_nnz) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::spgemm_batched_symbolic(&hs->h, it, vt, A, a_row_off, B, b_row_off, count, c_row_ptr, c_nnz);
+}
+
+struct nsp_graph_s {
+  nsp::GraphPlan *gp;
+};
+
+int nsp_graph_create(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const int64_t *a_row_off,
+                     const nsp_csr_t *B, const int64_t *b_row_off, int64_t count, nsp_csr_t *C, nsp_graph_t *g) {
+  if (!hs || !g) return NSP_ERR_INVALID;
+  *g = nullptr;
+  nsp::GraphPlan *gp = nullptr;
+  int s = nsp::graph_create(&hs->h, it, vt, A, a_row_off, B, b_row_off, count, C, &gp);
+  if (s != NSP_OK) return s;
+  *g = new (std::nothrow) nsp_graph_s{gp};
+  if (!*g) {
+    nsp::graph_destroy(gp);
+    return NSP_ERR_OOM;
+  }
+  return NSP_OK;
+}
+
+int nsp_graph_launch(nsp_graph_t g) { return g ? nsp::graph_launch(g->gp) : NSP_ERR_INVALID; }
+
+int nsp_graph_result(nsp_graph_t g, int64_t *nnz) { return g ? nsp::graph_result(g->gp, nnz) : NSP_ERR_INVALID; }
+
+const char *nsp_graph_last_error(nsp_graph_t g) { return g ? nsp::graph_last_error(g->gp) : ""; }
+
+int nsp_graph_destroy(nsp_graph_t g) {
+  if (!g) return NSP_OK;
+  int s = nsp::graph_destroy(g->gp);
+  delete g;
+  return s;
+}
+
+int nsp_transpose(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp_csr_t *XT) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::transpose(&hs->h, it, vt, X, XT);
+}
+
+int nsp_spgemm_backward(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                        const nsp_csr_t *AT, const nsp_csr_t *C, const void *dC, void *dA, void *dB) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::spgemm_backward(&hs->h, it, vt, A, B, AT, C, dC, dA, dB);
 }
 
 int nsp_gather_rows_fill(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *B, int64_t nr,

@@ -33,7 +33,17 @@ struct Info {
   int64_t max_nz;
   uint32_t tile_ticket;
   uint32_t error;
-  int64_t pad[4];
+  int64_t cap;  // capacity of C's column/value buffers (numeric rows beyond it are skipped)
+  int64_t pad[3];
+};
+
+// A batch of independent products stored as concatenated rows: matrix b owns
+// rows [a_off[b], a_off[b+1]) of A and [b_off[b], b_off[b+1]) of B (device
+// arrays of count+1 entries); columns are local to each matrix.
+struct Batch {
+  const int64_t *a_off = nullptr;
+  const int64_t *b_off = nullptr;
+  int64_t count = 0;
 };
 
 struct Workspace {
@@ -63,6 +73,7 @@ struct Handle {
   // reusable device workspace (ws: product; aux: compaction and gathers)
   Workspace ws;
   Workspace aux;
+  Workspace aux2;  // transpose and backward
   Info *d_info = nullptr;
   Info *h_info = nullptr;  // pinned
 

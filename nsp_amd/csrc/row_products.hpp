@@ -48,14 +48,17 @@ struct Window {
   bool ok[R];
 };
 
-// f(key[R], product[R], ok[R]) is called once per window with lane g's R
-// products of the window; ok[u] is false past the end of the entries.  VAL =
-// false skips the values (symbolic pass; products are 0).  Within a group all
-// GS lanes must be active; different groups of a wave may be in different
-// iterations.  Entries are taken from chunks [c0, c0+GS), c0 = a0 + first,
-// first + step, ... (step = GS for a group that owns the row; a block splits
-// a row's chunks over its waves with first = wave*64, step = 64*waves).
-template <int GS, int R, bool VAL, typename V, typename F>
+// f(key[R], product[R], ok[R], bval[R], entry[R]) is called once per window
+// with lane g's R products of the window; ok[u] is false past the end of the
+// entries, entry[u] is the product's compacted entry index in the chunk.
+// VAL = true loads B values and forms a_ik*b_kj; LOADB = true loads B values
+// without A values (products are then 0); both false skips values (symbolic
+// pass).  Within a group all GS lanes must be active; different groups of a
+// wave may be in different iterations.  Entries are taken from chunks
+// [c0, c0+GS), c0 = a0 + first, first + step, ... (step = GS for a group that
+// owns the row; a block splits a row's chunks over its waves with
+// first = wave*64, step = 64*waves).
+template <int GS, int R, bool VAL, typename V, bool LOADB = VAL, typename F>
 __device__ __forceinline__ void row_products(const int glane, const int64_t a0, const int64_t a1,
                                              const int64_t first, const int64_t step,
                                              const int64_t *__restrict__ ab_b0,
@@ -108,7 +111,7 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
           const int e = W.ok[u] ? before + __popcll(es->mask[q] & le) - 1 : 0;
           const int64_t jb = es->base[e] + (W.ok[u] ? sw + q * GS + glane : 0);
           W.key[u] = bcol[jb];
-          if (VAL) W.bv[u] = bval[jb];
+          if (VAL || LOADB) W.bv[u] = bval[jb];
           W.e[u] = e;
         }
       };
@@ -119,7 +122,7 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
         V prod[R];
 #pragma unroll
         for (int u = 0; u < R; u++) prod[u] = VAL ? es->av[cur.e[u]] * cur.bv[u] : (V)0;
-        f(cur.key, prod, cur.ok);
+        f(cur.key, prod, cur.ok, cur.bv, cur.e);
         cur = nxt;
       }
     }

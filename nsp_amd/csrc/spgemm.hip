@@ -43,11 +43,12 @@ constexpr uint8_t kNoBin = 255;
 // init: zero the per-call counters and the scan's tile status words.
 // (A fill kernel rather than hipMemsetAsync so the sequence stays valid inside
 // a captured graph.)
-__global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles) {
+__global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles, int64_t cap) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t *w = reinterpret_cast<int64_t *>(info);
   constexpr int nw = sizeof(Info) / sizeof(int64_t);
-  if (t < nw) w[t] = 0;
+  constexpr int wcap = (int)(offsetof(Info, cap) / sizeof(int64_t));
+  if (t < nw) w[t] = (t == wcap) ? cap : 0;
   for (int64_t i = t; i < ntiles; i += (int64_t)gridDim.x * blockDim.x) tile_status[i] = 0;
 }
 
@@ -57,10 +58,17 @@ __global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles) {
 // A entry (i,k) the start and length of B row k (ab_b0, ab_len), which the
 // hash passes then read with one independent load instead of the dependent
 // A.col -> B.row_ptr gather.  16 lanes per row so the A.col reads coalesce.
+//
+// Batched products (a_row_off != NULL): A and B hold `count` matrices as
+// concatenated rows with matrix-local columns; row i of matrix b selects B
+// row b_row_off[b] + k for its column k.  This is the only place the batch
+// structure enters: every later pass works on the concatenated rows.
 template <typename RowT>
 __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
                                                  const int32_t *__restrict__ acol,
                                                  const RowT *__restrict__ brpt,
+                                                 const int64_t *__restrict__ a_row_off,
+                                                 const int64_t *__restrict__ b_row_off, int64_t count,
                                                  int64_t *__restrict__ arpt64,
                                                  int64_t *__restrict__ ab_b0,
                                                  int32_t *__restrict__ ab_len,
@@ -78,9 +86,19 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
   for (int64_t i = blockIdx.x * (int64_t)(256 / GS) + threadIdx.x / GS; i < MA;
        i += (int64_t)gridDim.x * (256 / GS)) {
     const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
+    int64_t bbase = 0;
+    if (a_row_off) {
+      // matrix of row i: last b with a_row_off[b] <= i
+      int64_t lo = 0, hi = count;
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a_row_off[mid] <= i) lo = mid; else hi = mid;
+      }
+      bbase = b_row_off[lo];
+    }
     int64_t s = 0;
     for (int64_t p = p0 + glane; p < p1; p += GS) {
-      const int32_t k = acol[p];
+      const int64_t k = bbase + acol[p];
       const int64_t b0 = (int64_t)brpt[k];
       const int64_t len = (int64_t)brpt[k + 1] - b0;
       ab_b0[p] = b0;
@@ -233,7 +251,7 @@ __global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64
       int cnt = 0;
       row_products<GS, R, false, float>(
           glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[slot],
-          [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R]) {
+          [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
             cnt += hash_insert_keys<LOG, R>(K, key, ok, glane);
           });
       cnt = group_sum<GS>(cnt);
@@ -269,7 +287,7 @@ __global__ __launch_bounds__(BS) void k_sym_block(int bin_lo, int bin_hi, int64_
     int cnt = 0;
     row_products<64, R, false, float>(
         lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[w],
-        [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R]) {
+        [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
           cnt += hash_insert_keys<LOG, R>(K, key, ok, lane);
         });
     cnt = group_sum<64>(cnt);
@@ -390,12 +408,16 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
     const int64_t t = base + slot;
     if (t < count) {
       const int32_t r = perm ? perm[off + t] : (int32_t)t;
+      if (crpt[r + 1] > info->cap) {  // C buffers too small: skip the row, flag it
+        if (glane == 0) atomicOr((unsigned int *)&const_cast<Info *>(info)->error, (unsigned int)NSP_ERR_OVERFLOW);
+        continue;
+      }
       clear_lds<GS, SHD>(K, glane, kEmpty);
       clear_lds<GS, SHD>(W, glane, 0);
       wave_lds_sync();
       row_products<GS, R, true, V>(glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, aval, bcol, bval,
                                    &ent[slot],
-                                   [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
+                                   [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
                                      hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, glane);
                                    });
       wave_lds_sync();
@@ -459,12 +481,17 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int64_t t = blockIdx.x; t < count; t += gridDim.x) {
     const int32_t r = perm ? perm[off + t] : (int32_t)t;
+    if (crpt[r + 1] > info->cap) {  // C buffers too small: skip the row, flag it
+      if (threadIdx.x == 0)
+        atomicOr((unsigned int *)&const_cast<Info *>(info)->error, (unsigned int)NSP_ERR_OVERFLOW);
+      continue;
+    }
     clear_lds<BS, SH + 64>(K, threadIdx.x, kEmpty);
     clear_lds<BS, SH + 64>(W, threadIdx.x, 0);
     __syncthreads();
     row_products<64, R, true, V>(lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, ab_b0, ab_len, aval, bcol, bval,
                                  &ent[w],
-                                 [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
+                                 [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
                                    hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, lane);
                                  });
     __syncthreads();
@@ -710,28 +737,29 @@ static int read_info(Handle *h) {
 
 // Row info + symbolic binning (no host sync).  Also used by nsp_intprod.
 template <typename RowT>
-static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B) {
+static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const Batch &bt = Batch{},
+                       int64_t cap = INT64_MAX) {
   const int64_t MA = A->rows;
   NSP_TRY(ensure_workspace(h, MA, A->nnz));
   timing_mark(h, 0);
   hipLaunchKernelGGL(k_init, dim3(std::max<int64_t>(1, std::min<int64_t>((h->ntiles + 255) / 256, 1024))),
-                     dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles);
+                     dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles, cap);
   const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 15) / 16, (int64_t)h->num_cu * 8));
   hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr,
-                     A->col, (const RowT *)B->row_ptr, h->a_rpt64, h->ab_b0, h->ab_len, h->intprod,
-                     h->row_nz, h->cls, h->d_info);
+                     A->col, (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->a_rpt64, h->ab_b0,
+                     h->ab_len, h->intprod, h->row_nz, h->cls, h->d_info);
   NSP_HIP(h, hipGetLastError());
   return NSP_OK;
 }
 
 template <typename RowT>
 static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void *c_row_ptr,
-                         int64_t *c_nnz) {
+                         int64_t *c_nnz, const Batch &bt = Batch{}) {
   const int64_t MA = A->rows;
   h->stats = nsp_stats_t{};
   h->have_plan = false;
   h->timing_pending = false;
-  NSP_TRY(run_rowinfo<RowT>(h, A, B));
+  NSP_TRY(run_rowinfo<RowT>(h, A, B, bt));
   NSP_TRY(read_info(h));
   const Info &hi = *h->h_info;
   if (hi.count_sym[kHeavyBin] > 0)
@@ -831,6 +859,176 @@ int spgemm_numeric(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A
   if (vt == NSP_F64) return numeric_impl<double>(h, A, B, C);
   if (vt == NSP_F32) return numeric_impl<float>(h, A, B, C);
   return set_error(h, NSP_ERR_INVALID, "unknown value type");
+}
+
+// ---------------------------------------------------------------------------
+// Batched products: the plain two-phase path over concatenated rows.
+static int check_batch_args(Handle *h, const nsp_csr_t *A, const int64_t *a_off, const nsp_csr_t *B,
+                            const int64_t *b_off, int64_t count) {
+  NSP_TRY(check_csr_args(h, "A", A));
+  NSP_TRY(check_csr_args(h, "B", B));
+  if (count < 0 || (count > 0 && (!a_off || !b_off)))
+    return set_error(h, NSP_ERR_INVALID, "batched product: bad matrix offsets");
+  if (count == 0 && A->rows > 0) return set_error(h, NSP_ERR_INVALID, "batched product: rows but no matrices");
+  return NSP_OK;
+}
+
+int spgemm_batched_symbolic(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const int64_t *a_off,
+                            const nsp_csr_t *B, const int64_t *b_off, int64_t count, void *c_row_ptr,
+                            int64_t *c_nnz) {
+  NSP_TRY(check_batch_args(h, A, a_off, B, b_off, count));
+  if (vt != NSP_F64 && vt != NSP_F32) return set_error(h, NSP_ERR_INVALID, "unknown value type");
+  if (!c_row_ptr || !c_nnz) return set_error(h, NSP_ERR_INVALID, "c_row_ptr / c_nnz is NULL");
+  const Batch bt{a_off, b_off, count};
+  if (it == NSP_IDX32) return symbolic_impl<int32_t>(h, A, B, c_row_ptr, c_nnz, bt);
+  if (it == NSP_IDX64) return symbolic_impl<int64_t>(h, A, B, c_row_ptr, c_nnz, bt);
+  return set_error(h, NSP_ERR_INVALID, "unknown index type");
+}
+
+// ---------------------------------------------------------------------------
+// Graph-captured product: binning, every bin's symbolic kernel, the scan, and
+// every bin's numeric kernel recorded once into a hipGraph on a private
+// workspace, with no host synchronisation inside.  Each kernel reads its
+// bin's row count from device memory and walks it with a grid-stride loop
+// sized for the worst case, so the graph stays valid for any pattern of the
+// same row and nnz counts.  C's column/value buffers have a fixed capacity;
+// rows that would pass it are skipped and reported.
+struct GraphPlan {
+  Handle hh;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t launch_stream = nullptr;
+  int64_t rows = 0;
+};
+
+template <typename RowT, typename V>
+static int graph_record(Handle *g, const nsp_csr_t *A, const nsp_csr_t *B, const Batch &bt, nsp_csr_t *C) {
+  const int64_t MA = A->rows;
+  NSP_TRY(run_rowinfo<RowT>(g, A, B, bt, C->nnz));
+  launch_scatter(g, MA, g->d_info->count_sym, g->d_info->cursor_sym, g->perm_sym);
+  for (int b = 0; b < kHeavyBin; b++)
+    launch_sym_bin(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, g->ab_len, B->col);
+  hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)(MA / kScanTile + 1)), dim3(kScanBS), 0, g->stream, MA, g->row_nz,
+                     (RowT *)C->row_ptr, g->c_rpt64, g->cls, g->tile_status, g->d_info);
+  launch_scatter(g, MA, g->d_info->count_num, g->d_info->cursor_num, g->perm_num);
+  for (int b = 0; b < kHeavyBin; b++)
+    launch_num_bin<V>(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len,
+                      B->col, (const V *)B->val, C->col, (V *)C->val);
+  NSP_HIP(g, hipGetLastError());
+  return NSP_OK;
+}
+
+int graph_create(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const int64_t *a_off,
+                 const nsp_csr_t *B, const int64_t *b_off, int64_t count, nsp_csr_t *C, GraphPlan **out) {
+  if (!out || !C || !C->row_ptr) return set_error(h, NSP_ERR_INVALID, "graph: bad arguments");
+  if (a_off) {
+    NSP_TRY(check_batch_args(h, A, a_off, B, b_off, count));
+  } else {
+    NSP_TRY(check_csr_args(h, "A", A));
+    NSP_TRY(check_csr_args(h, "B", B));
+    if (A->cols != B->rows) return set_error(h, NSP_ERR_INVALID, "A.cols != B.rows");
+  }
+  if (A->rows < 1) return set_error(h, NSP_ERR_INVALID, "graph: A has no rows");
+  if (C->nnz > 0 && (!C->col || !C->val)) return set_error(h, NSP_ERR_INVALID, "graph: C buffers are NULL");
+  if (it == NSP_IDX32 && C->nnz > INT32_MAX) return set_error(h, NSP_ERR_OVERFLOW, "graph: capacity needs NSP_IDX64");
+  GraphPlan *gp = new GraphPlan();
+  Handle *g = &gp->hh;
+  g->device = h->device;
+  g->num_cu = h->num_cu;
+  g->grid_cap = h->grid_cap;
+  g->sym_rows_per_wave = h->sym_rows_per_wave;
+  g->num_rows_per_wave = h->num_rows_per_wave;
+  g->num_table_factor = h->num_table_factor;
+  gp->rows = A->rows;
+  gp->launch_stream = h->stream;
+  auto fail = [&](int s) {
+    h->last_error = g->last_error;
+    if (gp->exec) (void)hipGraphExecDestroy(gp->exec);
+    if (gp->graph) (void)hipGraphDestroy(gp->graph);
+    if (g->ws.base) (void)hipFree(g->ws.base);
+    if (g->d_info) (void)hipFree(g->d_info);
+    if (g->h_info) (void)hipHostFree(g->h_info);
+    if (gp->cap_stream) (void)hipStreamDestroy(gp->cap_stream);
+    delete gp;
+    return s;
+  };
+  if (hipMalloc((void **)&g->d_info, sizeof(Info)) != hipSuccess ||
+      hipHostMalloc((void **)&g->h_info, sizeof(Info), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    g->last_error = "graph: allocation failed";
+    return fail(NSP_ERR_OOM);
+  }
+  if (hipStreamCreateWithFlags(&gp->cap_stream, hipStreamNonBlocking) != hipSuccess) return fail(NSP_ERR_HIP);
+  g->stream = gp->cap_stream;
+  // fix the workspace before capture: it must not move under the graph
+  {
+    int s = ensure_workspace(g, A->rows, A->nnz);
+    if (s != NSP_OK) return fail(s);
+  }
+  if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(NSP_ERR_HIP);
+  const Batch bt{a_off, b_off, a_off ? count : 0};
+  int s;
+  if (it == NSP_IDX32) s = (vt == NSP_F64) ? graph_record<int32_t, double>(g, A, B, bt, C)
+                                           : graph_record<int32_t, float>(g, A, B, bt, C);
+  else s = (vt == NSP_F64) ? graph_record<int64_t, double>(g, A, B, bt, C)
+                           : graph_record<int64_t, float>(g, A, B, bt, C);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+  gp->graph = graph;
+  if (s != NSP_OK) return fail(s);
+  if (ec != hipSuccess || !graph) {
+    g->last_error = std::string("graph capture failed: ") + hipGetErrorString(ec);
+    return fail(NSP_ERR_HIP);
+  }
+  if (hipGraphInstantiate(&gp->exec, gp->graph, nullptr, nullptr, 0) != hipSuccess) {
+    g->last_error = "graph instantiation failed";
+    return fail(NSP_ERR_HIP);
+  }
+  *out = gp;
+  return NSP_OK;
+}
+
+int graph_launch(GraphPlan *gp) {
+  NSP_HIP(&gp->hh, hipGraphLaunch(gp->exec, gp->launch_stream));
+  return NSP_OK;
+}
+
+int graph_result(GraphPlan *gp, int64_t *nnz) {
+  Handle *g = &gp->hh;
+  NSP_HIP(g, hipMemcpyAsync(g->h_info, g->d_info, sizeof(Info), hipMemcpyDeviceToHost, gp->launch_stream));
+  NSP_HIP(g, hipStreamSynchronize(gp->launch_stream));
+  const Info &hi = *g->h_info;
+  if (nnz) *nnz = hi.nnz;
+  if (hi.count_sym[kHeavyBin] > 0 || hi.count_num[kHeavyBin] > 0)
+    return set_error(g, NSP_ERR_UNSUPPORTED, "graph: rows beyond 16384 products / 4096 entries need the plain path");
+  if (hi.error) return set_error(g, (int)hi.error, "graph: C exceeds the buffer capacity or the index width");
+  return NSP_OK;
+}
+
+const char *graph_last_error(GraphPlan *gp) { return gp->hh.last_error.c_str(); }
+
+int graph_destroy(GraphPlan *gp) {
+  if (!gp) return NSP_OK;
+  Handle *g = &gp->hh;
+  if (gp->launch_stream) (void)hipStreamSynchronize(gp->launch_stream);
+  if (gp->exec) (void)hipGraphExecDestroy(gp->exec);
+  if (gp->graph) (void)hipGraphDestroy(gp->graph);
+  if (g->ws.base) (void)hipFree(g->ws.base);
+  if (g->d_info) (void)hipFree(g->d_info);
+  if (g->h_info) (void)hipHostFree(g->h_info);
+  if (gp->cap_stream) (void)hipStreamDestroy(gp->cap_stream);
+  delete gp;
+  return NSP_OK;
+}
+
+int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B) {
+  NSP_TRY(check_csr_args(h, "A", A));
+  NSP_TRY(check_csr_args(h, "B", B));
+  if (A->cols != B->rows) return set_error(h, NSP_ERR_INVALID, "A.cols != B.rows");
+  h->have_plan = false;
+  if (it == NSP_IDX32) return run_rowinfo<int32_t>(h, A, B);
+  return run_rowinfo<int64_t>(h, A, B);
 }
 
 int spgemm_intprod(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, int64_t *intprod,

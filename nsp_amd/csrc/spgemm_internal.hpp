@@ -10,6 +10,22 @@ int spgemm_numeric(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A
 int spgemm_intprod(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, int64_t *intprod,
                    int64_t *nprod);
 int check_csr(Handle *h, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row);
+// Fills the handle's per-A-entry B row extents (ab_b0, ab_len) for A*B; ends
+// any symbolic plan held by the handle.
+int spgemm_batched_symbolic(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const int64_t *a_off,
+                            const nsp_csr_t *B, const int64_t *b_off, int64_t count, void *c_row_ptr,
+                            int64_t *c_nnz);
+struct GraphPlan;
+int graph_create(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const int64_t *a_off,
+                 const nsp_csr_t *B, const int64_t *b_off, int64_t count, nsp_csr_t *C, GraphPlan **out);
+int graph_launch(GraphPlan *gp);
+int graph_result(GraphPlan *gp, int64_t *nnz);
+const char *graph_last_error(GraphPlan *gp);
+int graph_destroy(GraphPlan *gp);
+int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B);
+int transpose(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp_csr_t *XT);
+int spgemm_backward(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                    const nsp_csr_t *AT, const nsp_csr_t *C, const void *dc, void *da, void *db);
 int shrink_cols(Handle *h, int64_t nnz, const int32_t *col, int64_t l, int64_t lo, int64_t hi,
                 int32_t *new_col, int32_t *perm, int64_t perm_cap, int64_t *shrinked_m);
 int gather_rows_count(Handle *h, nsp_index_t it, const nsp_csr_t *B, int64_t nr, const int32_t *rows,

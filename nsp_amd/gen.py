@@ -179,29 +179,48 @@ def random_csr(nrows: int, ncols: int, density: float, seed: int = 0,
     return _from_rows_cols(nrows, ncols, rows, cols, vals.astype(np.float64))
 
 
+def uneven_batch_concat(count: int = 4096, seed: int = 0, nmin: int = 64, nmax: int = 4096,
+                        lam: float = 8.0) -> tuple[HostCSR, np.ndarray]:
+    """`count` independent square matrices as one concatenated CSR (rows of
+    matrix 0, then matrix 1, ...; columns local to each matrix) and the row
+    offsets of the matrices (count+1 entries).  n_i ~ U[nmin, nmax]; entries
+    per row ~ max(Poisson(lam), 1) (capped at n_i); columns uniform without
+    replacement; values U(-1,1).  The concatenated CSR's ncols is the largest
+    n_i."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, count).astype(np.int64)
+    row_off = np.zeros(count + 1, np.int64)
+    np.cumsum(ns, out=row_off[1:])
+    M = int(row_off[-1])
+    nrow = np.repeat(ns, ns)                          # n of each row's matrix
+    k = np.minimum(np.maximum(rng.poisson(lam, M), 1), nrow)
+    rows = np.repeat(np.arange(M, dtype=np.int64), k)
+    nn = np.repeat(nrow, k)
+    cols = (rng.random(rows.size) * nn).astype(np.int64)
+    # redraw duplicates within a row until every row's columns are distinct
+    while True:
+        key = rows * (nmax + 1) + cols
+        order = np.argsort(key, kind="stable")
+        ks = key[order]
+        dup = np.zeros(rows.size, bool)
+        dup[order[1:]] = ks[1:] == ks[:-1]
+        nd = int(dup.sum())
+        if nd == 0:
+            break
+        cols[dup] = (rng.random(nd) * nn[dup]).astype(np.int64)
+    vals = rng.uniform(-1.0, 1.0, rows.size)
+    X = _from_rows_cols(M, int(ns.max()) if count else 0, rows, cols, vals)
+    return X, row_off
+
+
 def uneven_batch(count: int = 4096, seed: int = 0, nmin: int = 64, nmax: int = 4096,
                  lam: float = 8.0) -> list[HostCSR]:
-    """`count` independent square matrices: n_i ~ U[nmin, nmax]; entries per
-    row ~ max(Poisson(lam), 1) (capped at n_i); columns uniform without
-    replacement; values U(-1,1)."""
-    rng = np.random.default_rng(seed)
-    ns = rng.integers(nmin, nmax + 1, count)
+    """The matrices of uneven_batch_concat as separate CSRs."""
+    X, off = uneven_batch_concat(count, seed, nmin, nmax, lam)
     out = []
-    for n in ns:
-        n = int(n)
-        k = np.minimum(np.maximum(rng.poisson(lam, n), 1), n)
-        rows = np.repeat(np.arange(n, dtype=np.int64), k)
-        cols = rng.integers(0, n, rows.size)
-        # redraw duplicates within a row until every row's columns are distinct
-        while True:
-            key = rows * n + cols
-            order = np.argsort(key, kind="stable")
-            dup = np.zeros(rows.size, bool)
-            ks = key[order]
-            dup[order[1:]] = ks[1:] == ks[:-1]
-            if not dup.any():
-                break
-            cols[dup] = rng.integers(0, n, int(dup.sum()))
-        vals = rng.uniform(-1.0, 1.0, rows.size)
-        out.append(_from_rows_cols(n, n, rows, cols, vals))
+    for b in range(count):
+        r0, r1 = int(off[b]), int(off[b + 1])
+        m = X.row_slice(r0, r1)
+        m.ncols = r1 - r0
+        out.append(m)
     return out

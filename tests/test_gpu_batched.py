@@ -1,0 +1,130 @@
+"""Batched products (concatenated CSR) and the hipGraph-captured product,
+against the oracle and against the plain per-matrix path.
+
+Graph replays are checked one by one, with eager GPU work and host
+synchronisations between them (a captured sequence must give the same result
+on every replay, not only the first)."""
+import numpy as np
+import pytest
+
+from nsp_amd import gen
+from tests.helpers import check_against_oracle, to_dtype
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def nsp():
+    import torch
+    assert torch.cuda.is_available()
+    import nsp_amd
+    return nsp_amd
+
+
+def dev_off(off):
+    import torch
+    return torch.from_numpy(np.asarray(off, np.int64)).cuda()
+
+
+def oracle_batch(orc, X, off):
+    """Per-matrix oracle products stitched into the concatenated layout."""
+    rpts, cols, vals, abss = [np.zeros(1, np.int64)], [], [], []
+    base = 0
+    for b in range(len(off) - 1):
+        m = X.row_slice(int(off[b]), int(off[b + 1]))
+        m.ncols = int(off[b + 1] - off[b])
+        r, c, v, a = orc.spgemm(m, m)
+        rpts.append(r[1:] + base)
+        base += int(r[-1])
+        cols.append(c)
+        vals.append(v)
+        abss.append(a)
+    return (np.concatenate(rpts), np.concatenate(cols), np.concatenate(vals), np.concatenate(abss))
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_batched_matches_oracle(nsp, orc, dtype):
+    print("case: batched", dtype, flush=True)
+    X, off = gen.uneven_batch_concat(48, seed=5, nmin=16, nmax=700)
+    X = to_dtype(X, dtype)
+    h = nsp.Handle()
+    dX = nsp.DeviceCSR.from_host(X, dtype)
+    d_off = dev_off(off)
+    C = h.batched_spgemm(dX, d_off, dX, d_off)
+    check_against_oracle(X, X, C, dtype, orc, ref=oracle_batch(orc, X, off))
+
+
+def test_batched_equals_per_matrix_loop(nsp):
+    print("case: batched vs loop", flush=True)
+    X, off = gen.uneven_batch_concat(32, seed=9, nmin=64, nmax=400)
+    X = to_dtype(X, "f32")
+    h = nsp.Handle()
+    dX = nsp.DeviceCSR.from_host(X, "f32")
+    C = h.batched_spgemm(dX, dev_off(off), dX, dev_off(off))
+    crpt = C.row_ptr.cpu().numpy()
+    ccol = C.col.cpu().numpy()
+    cval = C.val.cpu().numpy()
+    for b in range(len(off) - 1):
+        m = X.row_slice(int(off[b]), int(off[b + 1]))
+        m.ncols = int(off[b + 1] - off[b])
+        dm = nsp.DeviceCSR.from_host(m, "f32")
+        Cb = h.spgemm(dm, dm)
+        r0, r1 = int(off[b]), int(off[b + 1])
+        s0, s1 = int(crpt[r0]), int(crpt[r1])
+        np.testing.assert_array_equal(Cb.row_ptr.cpu().numpy(), crpt[r0:r1 + 1] - s0)
+        np.testing.assert_array_equal(Cb.col.cpu().numpy(), ccol[s0:s1])
+        np.testing.assert_allclose(Cb.val.cpu().numpy(), cval[s0:s1], rtol=1e-5, atol=1e-6)
+
+
+def test_graph_single_product_replays(nsp, orc):
+    print("case: graph single", flush=True)
+    import torch
+    A = gen.stencil_27pt(18, seed=4)
+    h = nsp.Handle()
+    dA = nsp.DeviceCSR.from_host(A)
+    ref = orc.spgemm(A, A)
+    g = h.graph(dA, dA, capacity=int(ref[0][-1]) + 100)
+    for rep in range(3):
+        g.launch()
+        C = g.result()
+        check_against_oracle(A, A, C, "f64", orc, ref=ref)
+        # eager work between replays
+        x = torch.randn(1000, device="cuda")
+        (x * 2).sum().item()
+        h.spgemm(dA, dA)
+        torch.cuda.synchronize()
+
+
+def test_graph_batched_replays_and_value_updates(nsp, orc):
+    print("case: graph batched", flush=True)
+    import torch
+    X, off = gen.uneven_batch_concat(40, seed=12, nmin=32, nmax=300)
+    X = to_dtype(X, "f32")
+    h = nsp.Handle()
+    dX = nsp.DeviceCSR.from_host(X, "f32")
+    d_off = dev_off(off)
+    ref = oracle_batch(orc, X, off)
+    g = h.graph(dX, dX, capacity=int(ref[0][-1]), a_off=d_off, b_off=d_off)
+    for rep in range(3):
+        g.launch()
+        C = g.result()
+        check_against_oracle(X, X, C, "f32", orc, ref=ref)
+        h.spgemm(nsp.DeviceCSR.from_host(gen.laplacian_5pt(20)), nsp.DeviceCSR.from_host(gen.laplacian_5pt(20)))
+        torch.cuda.synchronize()
+    # new values, same pattern: the same graph computes the new product
+    X2 = gen.HostCSR(X.rpt, X.col, (-2.0 * X.val).astype(np.float32).astype(np.float64), X.nrows, X.ncols)
+    dX.val.copy_(torch.from_numpy(X2.val).to(dX.val.dtype))
+    g.launch()
+    check_against_oracle(X2, X2, g.result(), "f32", orc, ref=oracle_batch(orc, X2, off))
+
+
+def test_graph_capacity_overflow_reported(nsp):
+    print("case: graph capacity", flush=True)
+    A = gen.stencil_27pt(10, seed=1)
+    h = nsp.Handle()
+    dA = nsp.DeviceCSR.from_host(A)
+    g = h.graph(dA, dA, capacity=1000)
+    g.launch()
+    with pytest.raises(nsp.NspError) as e:
+        g.result()
+    assert e.value.status == 3

@@ -359,3 +359,18 @@ def test_planted_wrong_shrink_branch_is_caught(orc):
     perm, newcol = orc.shrink(col, 9)
     wrong_perm, _ = shrink_branch_form(col, 9, "first")   # first-rank form on mixed halos
     assert not np.array_equal(perm, wrong_perm)
+
+
+def test_uneven_batch_generator():
+    X, off = gen.uneven_batch_concat(20, seed=3, nmin=64, nmax=300)
+    assert off[0] == 0 and off[-1] == X.nrows and len(off) == 21
+    sizes = np.diff(off)
+    assert sizes.min() >= 64 and sizes.max() <= 300
+    for b in range(20):
+        m = X.row_slice(int(off[b]), int(off[b + 1]))
+        assert m.col.max() < sizes[b]
+        for i in range(m.nrows):
+            seg = m.col[m.rpt[i]:m.rpt[i + 1]]
+            assert seg.size >= 1 and np.all(np.diff(seg) > 0)
+    mats = gen.uneven_batch(20, seed=3, nmin=64, nmax=300)
+    np.testing.assert_array_equal(mats[7].col, X.row_slice(int(off[7]), int(off[8])).col)
