@@ -94,6 +94,116 @@ def cpu_baseline(A, B, nprod_total, budget_s=20.0):
                       f"row-wise Gustavson with a dense accumulator, OpenMP {threads} threads"}
 
 
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def fwd_bwd_leg(dtype: str, m: int = 64, steps: int = 50, warmup: int = 5):
+    """27-point stencil m^3 times the aggregation prolongator (n x n/8) on
+    fixed patterns: forward = numeric phase on the cached pattern, backward =
+    loss gradient dC = C - T, A^T value gather, dA and dB masked products."""
+    import torch
+    import nsp_amd
+    from nsp_amd import gen
+    from nsp_amd.autograd import SpGEMMFixedPattern
+    A = gen.stencil_27pt(m, seed=0)
+    P = gen.aggregation(A.nrows, seed=1)
+    dA = nsp_amd.DeviceCSR.from_host(A, dtype)
+    dP = nsp_amd.DeviceCSR.from_host(P, dtype)
+    op = SpGEMMFixedPattern(dA, dP)
+    T = torch.randn(op.C.nnz, dtype=dA.val.dtype, device="cuda",
+                    generator=torch.Generator("cuda").manual_seed(3))
+    a, b = dA.val, dP.val
+    hf = nsp_amd.Handle()
+
+    def full():
+        r, z = hf.symbolic(dA, dP)
+        hf.numeric(dA, dP, r, z)
+
+    def fwd():
+        op.forward_values(a, b)
+
+    def fwd_bwd():
+        c = op.forward_values(a, b)
+        op.backward_values(a, b, c - T)
+
+    t_full = _timed(full, steps, warmup)
+    t_fwd = _timed(fwd, steps, warmup)
+    t_fb = _timed(fwd_bwd, steps, warmup)
+    nprod = op.h.stats()["nprod"]
+    return {"workload": f"27-point stencil {m}^3 x aggregation P ({A.nrows}x{P.ncols}), {dtype}",
+            "nnz_A": A.nnz, "nnz_P": P.nnz, "nnz_C": op.C.nnz, "nprod": nprod,
+            "fwd_ms": round(t_fwd, 4), "fwd_bwd_ms": round(t_fb, 4),
+            "ratio": round(t_fb / t_fwd, 3), "fwd_with_symbolic_ms": round(t_full, 4),
+            "ratio_vs_fwd_with_symbolic": round(t_fb / t_full, 3),
+            "note": "forward on the cached pattern = numeric phase; backward = dC, A^T value "
+                    "gather, dA and dB (2 host syncs)"}
+
+
+def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int = 2):
+    """`count` independent FP32 products C_i = A_i*A_i of uneven sizes: one
+    hipGraph launch over the concatenated batch vs a per-matrix loop of the
+    plain two-phase product on the same GPU; results compared entry by
+    entry."""
+    import torch
+    import nsp_amd
+    from nsp_amd import gen
+    X, off = gen.uneven_batch_concat(count, seed=0)
+    dX = nsp_amd.DeviceCSR.from_host(X, "f32")
+    d_off = torch.from_numpy(off).cuda()
+    h = nsp_amd.Handle()
+    _, nnz = h.batched_symbolic(dX, d_off, dX, d_off)
+    nprod = h.stats()["nprod"]
+    g = h.graph(dX, dX, capacity=nnz, a_off=d_off, b_off=d_off)
+    t_graph = _timed(g.launch, steps, warmup)
+    Cg = g.result()
+    t_eager = _timed(lambda: h.batched_spgemm(dX, d_off, dX, d_off), 2, 1)
+
+    mats = []
+    for b in range(count):
+        r0, r1 = int(off[b]), int(off[b + 1])
+        s0, s1 = int(X.rpt[r0]), int(X.rpt[r1])
+        rp = (dX.row_ptr[r0:r1 + 1] - s0).contiguous()
+        mats.append(nsp_amd.DeviceCSR(rp, dX.col[s0:s1], dX.val[s0:s1], r1 - r0, r1 - r0))
+    hl = nsp_amd.Handle()
+    outs = []
+
+    def loop():
+        outs.clear()
+        outs.extend(hl.spgemm(mt, mt) for mt in mats)
+
+    t_loop = _timed(loop, loops, 1)
+    # same structure, values compared componentwise (|C_graph - C_loop| over
+    # the graph's product of |A_i|, |A_i|)
+    col = torch.cat([o.col for o in outs])
+    val = torch.cat([o.val for o in outs])
+    nnz_each = torch.tensor([o.nnz for o in outs], dtype=torch.int64)
+    gr = Cg.row_ptr.to(torch.int64).cpu()
+    same = (col.numel() == Cg.nnz and torch.equal(col, Cg.col)
+            and torch.equal(nnz_each, gr[torch.from_numpy(off[1:])] - gr[torch.from_numpy(off[:-1])]))
+    vg = Cg.val.clone()
+    saved = dX.val.clone()
+    dX.val.abs_()
+    g.launch()
+    absC = g.result().val.clone()
+    dX.val.copy_(saved)
+    err = float(((vg - val).abs() / absC.clamp_min(1e-30)).max()) if same and nnz else None
+    return {"workload": f"{count} uneven FP32 products A_i*A_i, n_i~U[64,4096], Poisson(8) entries/row",
+            "rows": X.nrows, "nnz": X.nnz, "nnz_C": nnz, "nprod": nprod,
+            "graph_ms": round(t_graph, 3), "eager_batched_ms": round(t_eager, 3),
+            "loop_ms": round(t_loop, 3), "speedup_vs_loop": round(t_loop / t_graph, 2),
+            "graph_GFLOPs": round(2.0 * nprod / t_graph / 1e6, 1),
+            "structure_equal": bool(same), "max_componentwise_diff": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +211,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="fwd_bwd,batched",
+                    help="extra measurements after the headline: fwd_bwd, batched (comma list, or none)")
+    ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--param", action="append", default=[],
                     help="library tuning parameter name=value (repeatable)")
     args = ap.parse_args()
@@ -177,8 +290,14 @@ def main():
         "host_syncs_per_step": st["host_syncs"],
         "params": params,
         "bins": {"sym": st["sym_bin"], "num": st["num_bin"], "sym_launches": st["sym_launches"],
-                 "num_launches": st["num_launches"], "sub_lanes": st["sub_lanes"]},
+                 "num_launches": st["num_launches"], "direct_mode": st["direct_mode"]},
     }
+    del C, dA, dB, h
+    legs = [x for x in args.legs.split(",") if x and x != "none"]
+    if "fwd_bwd" in legs:
+        out["fwd_bwd"] = {dt: fwd_bwd_leg(dt) for dt in ("f64", "f32")}
+    if "batched" in legs:
+        out["batched"] = batched_leg(args.batch_count)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, B, nprod)
     print(json.dumps(out), flush=True)

@@ -65,7 +65,8 @@ typedef struct {
   int64_t sym_bin[16];      /* rows per symbolic bin */
   int64_t num_bin[16];      /* rows per numeric bin */
   int32_t n_sym_bins, n_num_bins;
-  int32_t sub_lanes;        /* lanes per A entry used by the product loops */
+  int32_t direct_mode;      /* bit 0: symbolic pass walked all rows in natural
+                             * order with one kernel, bit 1: numeric pass did */
   int32_t host_syncs;       /* blocking host synchronisations in the call */
   int32_t sym_launches;     /* symbolic hash kernels launched */
   int32_t num_launches;     /* numeric hash kernels launched */
@@ -85,15 +86,16 @@ int nsp_set_stream(nsp_handle_t h, void *hip_stream);
 const char *nsp_status_string(int status);
 const char *nsp_last_error(nsp_handle_t h);
 int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
-/* Tuning knobs (an autotuner sets these):
- *   "sub_lanes"    lanes per A entry in the product loops (0 = from avg row
- *                  length of B, else 1, 4 or 16)
- *   "sym_factor"   symbolic hash table slots per intermediate product (1, 2)
- *   "grid_cap"     blocks per CU for persistent launches
- *   "sym_rows_per_wave"  rows per wavefront in the symbolic bins of 129..1024
- *                  products (1, 2, 4)
+/* Tuning knobs (nsp_amd.autotune picks them per workload):
+ *   "grid_cap"     blocks per CU for persistent launches (1..64)
+ *   "sym_small_rows_per_wave"  rows per wavefront in the symbolic bins of
+ *                  up to 128 products (1, 2, 4)
+ *   "sym_rows_per_wave"  rows per wavefront in the symbolic bins of
+ *                  129..1024 products (1, 2, 4)
+ *   "num_small_rows_per_wave"  rows per wavefront in the numeric bins of up
+ *                  to 64 entries (1, 2, 4)
  *   "num_rows_per_wave"  rows per wavefront in the numeric bins of 65..256
- *                  entries (1, 2)
+ *                  entries (1, 2, 4)
  *   "num_table_factor"   numeric hash slots per row entry (2, 4)
  *   "timing"       1 records HIP events around each phase (see nsp_stats_t) */
 int nsp_set_param(nsp_handle_t h, const char *name, int64_t value);
@@ -189,11 +191,14 @@ int nsp_transpose(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_
  * C's pattern (C->row_ptr, C->col from the forward; dC aligned with C->col):
  *   dA = (dC B^T) sampled on pat(A)   (dA aligned with A->col)
  *   dB = (A^T dC) sampled on pat(B)   (dB aligned with B->col)
- * AT is A's transpose from nsp_transpose.  Deterministic.  No reference
+ * AT is A's transpose from nsp_transpose.  at_perm (optional, index type
+ * `it`, AT->nnz entries) maps each A^T entry to its A entry; when it is given
+ * A's values are read as A->val[at_perm[t]] and AT->val may be NULL, so a
+ * cached A^T pattern serves any values of A.  Deterministic.  No reference
  * counterpart (the reference has no backward); C rows of up to 1024 entries. */
 int nsp_spgemm_backward(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A,
-                        const nsp_csr_t *B, const nsp_csr_t *AT, const nsp_csr_t *C,
-                        const void *dC, void *dA, void *dB);
+                        const nsp_csr_t *B, const nsp_csr_t *AT, const void *at_perm,
+                        const nsp_csr_t *C, const void *dC, void *dA, void *dB);
 
 /* ---- one rank's local share on one device ------------------------------- */
 /* Column compaction ("shrink"): perm = sorted unique(col U [0..l]),

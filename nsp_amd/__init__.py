@@ -67,7 +67,7 @@ class DeviceCSR:
     def c_struct(self) -> _lib.nsp_csr_t:
         return _lib.nsp_csr_t(self.nrows, self.ncols, self.nnz, self.row_ptr.data_ptr(),
                               self.col.data_ptr() if self.nnz else None,
-                              self.val.data_ptr() if self.nnz else None)
+                              self.val.data_ptr() if self.nnz and self.val is not None else None)
 
 
 class Handle:
@@ -109,7 +109,7 @@ class Handle:
         self._check(self.L.nsp_get_stats(self.h, C.byref(st)))
         return {"nprod": st.nprod, "nnz": st.nnz, "max_intprod": st.max_intprod,
                 "max_row_nnz": st.max_row_nnz, "sym_bin": list(st.sym_bin[:st.n_sym_bins]),
-                "num_bin": list(st.num_bin[:st.n_num_bins]), "sub_lanes": st.sub_lanes,
+                "num_bin": list(st.num_bin[:st.n_num_bins]), "direct_mode": st.direct_mode,
                 "host_syncs": st.host_syncs, "sym_launches": st.sym_launches,
                 "num_launches": st.num_launches, "t_bin_ms": st.t_bin_ms,
                 "t_sym_ms": st.t_sym_ms, "t_scan_ms": st.t_scan_ms, "t_num_ms": st.t_num_ms}
@@ -188,14 +188,20 @@ class Handle:
         self._check(self.L.nsp_transpose(self.h, X.index_type, X.value_type, C.byref(x), C.byref(xt)))
         return XT
 
-    def backward(self, A: DeviceCSR, B: DeviceCSR, AT: DeviceCSR, Cpat: DeviceCSR, dC):
-        """(dA, dB) aligned with A.col and B.col for dC aligned with Cpat.col."""
+    def backward(self, A: DeviceCSR, B: DeviceCSR, AT: DeviceCSR, Cpat: DeviceCSR, dC, at_perm=None):
+        """(dA, dB) aligned with A.col and B.col for dC aligned with Cpat.col.
+        With at_perm (A^T entry -> A entry, A's index dtype) A's values come
+        from A.val through it and AT.val is not read."""
         torch = _torch()
         dA = torch.empty(A.nnz, dtype=A.val.dtype, device=A.col.device)
         dB = torch.empty(B.nnz, dtype=B.val.dtype, device=B.col.device)
         a, b, at, c = A.c_struct(), B.c_struct(), AT.c_struct(), Cpat.c_struct()
+        if at_perm is not None:
+            if at_perm.dtype != A.row_ptr.dtype or at_perm.numel() != A.nnz:
+                raise ValueError("at_perm must have A.nnz entries of A's index dtype")
+        perm_p = C.c_void_p(at_perm.data_ptr() if at_perm is not None and at_perm.numel() else None)
         self._check(self.L.nsp_spgemm_backward(
-            self.h, A.index_type, A.value_type, C.byref(a), C.byref(b), C.byref(at), C.byref(c),
+            self.h, A.index_type, A.value_type, C.byref(a), C.byref(b), C.byref(at), perm_p, C.byref(c),
             C.c_void_p(dC.data_ptr() if dC.numel() else None),
             C.c_void_p(dA.data_ptr() if dA.numel() else None),
             C.c_void_p(dB.data_ptr() if dB.numel() else None)))

@@ -26,7 +26,7 @@ class nsp_stats_t(C.Structure):
     _fields_ = [("nprod", C.c_int64), ("nnz", C.c_int64), ("max_intprod", C.c_int64),
                 ("max_row_nnz", C.c_int64), ("sym_bin", C.c_int64 * 16),
                 ("num_bin", C.c_int64 * 16), ("n_sym_bins", C.c_int32),
-                ("n_num_bins", C.c_int32), ("sub_lanes", C.c_int32), ("host_syncs", C.c_int32),
+                ("n_num_bins", C.c_int32), ("direct_mode", C.c_int32), ("host_syncs", C.c_int32),
                 ("sym_launches", C.c_int32), ("num_launches", C.c_int32),
                 ("t_bin_ms", C.c_float), ("t_sym_ms", C.c_float), ("t_scan_ms", C.c_float),
                 ("t_num_ms", C.c_float)]
@@ -67,7 +67,7 @@ SIGNATURES = [
     ("nsp_graph_last_error", C.c_char_p, [_H]),
     ("nsp_graph_destroy", _I, [_H]),
     ("nsp_transpose", _I, [_H, _I, _I, _CSRP, _CSRP]),
-    ("nsp_spgemm_backward", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _CSRP, _VP, _VP, _VP]),
+    ("nsp_spgemm_backward", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _VP, _CSRP, _VP, _VP, _VP]),
 ]
 
 

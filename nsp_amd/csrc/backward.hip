@@ -162,9 +162,38 @@ int transpose(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp
 }
 
 // ---------------------------------------------------------------------------
-// dA: GS lanes per A row; C row i (nz <= NS) in an LDS table column -> dC.
+// LDS hash of one C row with a table size chosen per row (the smallest power
+// of two >= 2 nz, at least 16 slots), so short rows clear and probe small
+// tables.  Same 24-bit multiplicative hash as the forward tables.
+__device__ __forceinline__ uint32_t hash_slot_rt(int32_t key, int log) {
+  const uint32_t p = (uint32_t)__umul24((uint32_t)key, 0x9E3779u);
+  return (p & 0xFFFFFFu) >> (24 - log);
+}
+
+template <typename V>
+__device__ __forceinline__ V hash_lookup_rt(const int32_t *keys, const V *vals, int32_t key, int log) {
+  const uint32_t mask = (1u << log) - 1;
+  uint32_t h = hash_slot_rt(key, log);
+  for (uint32_t step = 0; step <= mask; step++) {  // bounded: a full table ends the walk
+    const int32_t k = keys[h];
+    if (k == key) return vals[h];
+    if (k == kEmpty) return (V)0;
+    h = (h + 1) & mask;
+  }
+  return (V)0;
+}
+
+__device__ __forceinline__ int table_log(int64_t nz) {
+  int log = 4;
+  while ((1ll << log) < 2 * nz) log++;
+  return log;
+}
+
+// dA: GS lanes per A row; C row i in an LDS table column -> dC.  A launch
+// handles the rows with nz_lo < nz(C_i) <= NS; rows longer than NS set
+// *too_long when too_long is given (the largest tier).
 template <typename V, typename RowT, int GS, int NS, int R>
-__global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const RowT *__restrict__ arpt,
+__global__ __launch_bounds__(256) void k_grad_a(int64_t MA, int64_t nz_lo, const RowT *__restrict__ arpt,
                                                 const int64_t *__restrict__ ab_b0,
                                                 const int32_t *__restrict__ ab_len,
                                                 const int32_t *__restrict__ bcol, const V *__restrict__ bval,
@@ -173,7 +202,6 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const RowT *__restri
                                                 uint32_t *__restrict__ too_long) {
   constexpr int RPB = 256 / GS;
   constexpr int SH = 2 * NS;
-  constexpr int LOG = Log2<SH>::value;
   __shared__ int32_t keys[RPB][SH];
   __shared__ V vals[RPB][SH];
   __shared__ V acc[RPB][GS];
@@ -183,16 +211,22 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const RowT *__restri
   V *W = vals[slot];
   for (int64_t i = blockIdx.x * (int64_t)RPB + slot; i < MA; i += (int64_t)gridDim.x * RPB) {
     const int64_t c0 = (int64_t)crpt[i], nz = (int64_t)crpt[i + 1] - c0;
+    if (nz <= nz_lo) continue;
     if (nz > NS) {
-      if (glane == 0) *too_long = 1;
+      if (too_long && glane == 0) *too_long = 1;
       continue;
     }
-    for (int s = glane; s < SH; s += GS) K[s] = kEmpty;
+    const int log = table_log(nz);
+    const uint32_t mask = (1u << log) - 1;
+    for (int s = glane; s <= (int)mask; s += GS) K[s] = kEmpty;
     wave_lds_sync();
     for (int64_t q = glane; q < nz; q += GS) {
       const int32_t key = ccol[c0 + q];
-      uint32_t hs = hash_slot<LOG>(key);
-      while (atomicCAS(&K[hs], kEmpty, key) != kEmpty) hs = (hs + 1) & (SH - 1);
+      uint32_t hs = hash_slot_rt(key, log);
+      for (uint32_t step = 0; step <= mask; step++) {  // nz < table size: a free slot exists
+        if (atomicCAS(&K[hs], kEmpty, key) == kEmpty) break;
+        hs = (hs + 1) & mask;
+      }
       W[hs] = dc[c0 + q];
     }
     wave_lds_sync();
@@ -202,16 +236,16 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const RowT *__restri
       acc[slot][glane] = (V)0;
       wave_lds_sync();
       row_products<GS, R, false, V, true>(glane, cc, (cc + GS < a1 ? cc + GS : a1), 0, GS, ab_b0, ab_len, nullptr,
-                                    bcol, bval, &ent[slot],
-                                    [&](const int32_t(&key)[R], const V(&)[R], const bool(&ok)[R],
-                                        const V(&bv)[R], const int(&e)[R]) {
+                                          bcol, bval, &ent[slot],
+                                          [&](const int32_t(&key)[R], const V(&)[R], const bool(&ok)[R],
+                                              const V(&bv)[R], const int(&e)[R]) {
 #pragma unroll
-                                      for (int u = 0; u < R; u++)
-                                        if (ok[u])
-                                          __hip_atomic_fetch_add(&acc[slot][e[u]],
-                                                                 bv[u] * hash_lookup<LOG, V>(K, W, key[u]),
-                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                    });
+                                            for (int u = 0; u < R; u++)
+                                              if (ok[u])
+                                                __hip_atomic_fetch_add(&acc[slot][e[u]],
+                                                                       bv[u] * hash_lookup_rt<V>(K, W, key[u], log),
+                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                          });
       wave_lds_sync();
       // map compacted entry index back to the entry's position
       const int64_t ja = cc + glane;
@@ -225,50 +259,63 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const RowT *__restri
   }
 }
 
-// dB: one GS-lane group per B row k; lanes walk the pairs (t, q) of A^T row
-// k's entries t and B row k's entries q; dC_ij found by binary search in the
-// sorted C row i.
-template <typename V, typename RowT, int GS>
+// dB: entry-parallel over B.  A block takes kGradBRows consecutive B rows and
+// stages their row pointers in LDS; sub-groups of SG lanes take the chunk's B
+// entries (k, j) in turn, and the SG lanes of a sub-group split A^T row k's
+// entries t (lane l takes t = l, l+SG, ...), each binary-searching dC_ij in
+// the sorted C row i.  Per-lane sums in t order, then a fixed shuffle tree:
+// deterministic.  A's value of A^T entry t is AT.val[t], or A.val[perm[t]]
+// when the permutation is given.
+constexpr int kGradBRows = 64;
+
+template <typename V, typename RowT, int SG>
 __global__ __launch_bounds__(256) void k_grad_b(int64_t MB, const RowT *__restrict__ brpt,
                                                 const int32_t *__restrict__ bcol, const RowT *__restrict__ atrpt,
                                                 const int32_t *__restrict__ atcol, const V *__restrict__ atval,
+                                                const V *__restrict__ aval, const RowT *__restrict__ at_perm,
                                                 const RowT *__restrict__ crpt, const int32_t *__restrict__ ccol,
                                                 const V *__restrict__ dc, V *__restrict__ db) {
-  constexpr int RPB = 256 / GS;
-  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
-  for (int64_t k = blockIdx.x * (int64_t)RPB + slot; k < MB; k += (int64_t)gridDim.x * RPB) {
-    const int64_t b0 = (int64_t)brpt[k], nb = (int64_t)brpt[k + 1] - b0;
-    const int64_t t0 = (int64_t)atrpt[k], nt = (int64_t)atrpt[k + 1] - t0;
-    // lanes take B entries q = glane, glane+GS, ...; each sums over all t in order
-    for (int64_t q = glane; q < nb; q += GS) {
-      const int32_t j = bcol[b0 + q];
+  constexpr int NSG = 256 / SG;
+  __shared__ int64_t s_rpt[kGradBRows + 1];
+  const int sg = threadIdx.x / SG, sl = threadIdx.x % SG;
+  for (int64_t k0 = blockIdx.x * (int64_t)kGradBRows; k0 < MB; k0 += (int64_t)gridDim.x * kGradBRows) {
+    const int nr = (int)(MB - k0 < kGradBRows ? MB - k0 : kGradBRows);
+    for (int q = threadIdx.x; q <= nr; q += 256) s_rpt[q] = (int64_t)brpt[k0 + q];
+    __syncthreads();
+    const int64_t e0 = s_rpt[0], e1 = s_rpt[nr];
+    // every sub-group runs the same number of rounds (the shuffle tree needs
+    // all lanes of the wavefront)
+    for (int64_t eb = e0; eb < e1; eb += NSG) {
+      const int64_t p = eb + sg;
+      const bool active = p < e1;
       V s = (V)0;
-      for (int64_t t = 0; t < nt; t++) {
-        const int32_t i = atcol[t0 + t];
-        const int64_t end = (int64_t)crpt[i + 1];
-        int64_t lo = (int64_t)crpt[i], hi = end;
-        while (lo < hi) {
-          const int64_t mid = (lo + hi) >> 1;
-          if (ccol[mid] < j) lo = mid + 1; else hi = mid;
+      if (active) {
+        int lo = 0, hi = nr;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_rpt[mid] <= p) lo = mid; else hi = mid;
         }
-        // j is in C row i whenever A_ik and B_kj are stored (C = A*B's pattern)
-        if (lo < end && ccol[lo] == j) s += atval[t0 + t] * dc[lo];
+        const int64_t k = k0 + lo;
+        const int32_t j = bcol[p];
+        const int64_t t0 = (int64_t)atrpt[k], t1 = (int64_t)atrpt[k + 1];
+        for (int64_t t = t0 + sl; t < t1; t += SG) {
+          const int32_t i = atcol[t];
+          const V a = at_perm ? aval[(int64_t)at_perm[t]] : atval[t];
+          const int64_t end = (int64_t)crpt[i + 1];
+          int64_t lo2 = (int64_t)crpt[i], hi2 = end;
+          while (lo2 < hi2) {
+            const int64_t mid = (lo2 + hi2) >> 1;
+            if (ccol[mid] < j) lo2 = mid + 1; else hi2 = mid;
+          }
+          // j is in C row i whenever A_ik and B_kj are stored (C = A*B's pattern)
+          if (lo2 < end && ccol[lo2] == j) s += a * dc[lo2];
+        }
       }
-      db[b0 + q] = s;
+      s = group_sum<SG>(s);
+      if (active && sl == 0) db[p] = s;
     }
+    __syncthreads();  // the next chunk reuses s_rpt
   }
-}
-
-// Largest row length of a CSR (for the dA table size), into *out (zeroed).
-template <typename RowT>
-__global__ void k_max_rowlen(int64_t rows, const RowT *__restrict__ rpt, unsigned long long *out) {
-  unsigned long long m = 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
-    const unsigned long long l = (unsigned long long)((int64_t)rpt[i + 1] - (int64_t)rpt[i]);
-    m = l > m ? l : m;
-  }
-  m = group_max<64>(m);
-  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
 __global__ void k_zero_words(uint32_t *p, int n) {
@@ -277,63 +324,71 @@ __global__ void k_zero_words(uint32_t *p, int n) {
 
 template <typename V, typename RowT>
 static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const nsp_csr_t *AT,
-                         const nsp_csr_t *C, const V *dc, V *da, V *db) {
+                         const RowT *at_perm, const nsp_csr_t *C, const V *dc, V *da, V *db) {
   hipStream_t st = h->stream;
   // per-entry B row extents: the binning pass of the forward
   NSP_TRY(spgemm_row_extents(h, sizeof(RowT) == 4 ? NSP_IDX32 : NSP_IDX64, A, B));
   NSP_TRY(aux2_reserve(h, 256));
-  uint32_t *flags = (uint32_t *)h->aux2.base;  // [0..1] max C row length, [2] too long
-  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 4);
-  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>((C->rows + 255) / 256, (int64_t)h->num_cu * 4));
-  hipLaunchKernelGGL((k_max_rowlen<RowT>), dim3(g1), dim3(256), 0, st, C->rows, (const RowT *)C->row_ptr,
-                     (unsigned long long *)flags);
-  unsigned long long maxnz = 0;
-  NSP_HIP(h, hipMemcpyAsync(&maxnz, flags, 8, hipMemcpyDeviceToHost, st));
-  NSP_HIP(h, hipStreamSynchronize(st));
-  uint32_t *too_long = flags + 2;
+  uint32_t *too_long = (uint32_t *)h->aux2.base;
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, too_long, 1);
   const int64_t MA = A->rows;
   if (MA > 0) {
-    if (maxnz <= 128) {
-      const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 8));
-      hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128, 4>), dim3(ga), dim3(256), 0, st, MA, (const RowT *)A->row_ptr,
-                         h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
-                         too_long);
-    } else if (maxnz <= 1024) {
-      const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 3) / 4, (int64_t)h->num_cu * 4));
-      hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024, 4>), dim3(ga), dim3(256), 0, st, MA,
-                         (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col, (const V *)B->val,
-                         (const RowT *)C->row_ptr, C->col, dc, da, too_long);
-    } else {
-      return set_error(h, NSP_ERR_UNSUPPORTED, "backward: C rows with more than 1024 entries");
-    }
+    // two tiers by C row length: <= 128 entries (32 lanes per row), 129..1024
+    // (64 lanes); each launch skips the other tier's rows
+    const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 8));
+    hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128, 4>), dim3(ga), dim3(256), 0, st, MA, (int64_t)-1,
+                       (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col, (const V *)B->val,
+                       (const RowT *)C->row_ptr, C->col, dc, da, (uint32_t *)nullptr);
+    const int gb = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 3) / 4, (int64_t)h->num_cu * 4));
+    hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024, 4>), dim3(gb), dim3(256), 0, st, MA, (int64_t)128,
+                       (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col, (const V *)B->val,
+                       (const RowT *)C->row_ptr, C->col, dc, da, too_long);
   }
-  if (B->rows > 0) {
-    const int gb = (int)std::max<int64_t>(1, std::min<int64_t>((B->rows + 15) / 16, (int64_t)h->num_cu * 8));
-    hipLaunchKernelGGL((k_grad_b<V, RowT, 16>), dim3(gb), dim3(256), 0, st, B->rows, (const RowT *)B->row_ptr,
-                       B->col, (const RowT *)AT->row_ptr, AT->col, (const V *)AT->val, (const RowT *)C->row_ptr,
-                       C->col, dc, db);
+  if (B->rows > 0 && B->nnz > 0) {
+    // lanes per B entry from the mean A^T row length
+    const int64_t avg = A->cols > 0 ? (A->nnz + A->cols - 1) / A->cols : 1;
+    const int g = (int)std::max<int64_t>(
+        1, std::min<int64_t>((B->rows + kGradBRows - 1) / kGradBRows, (int64_t)h->num_cu * 8));
+#define GRAD_B(SG)                                                                                           \
+  hipLaunchKernelGGL((k_grad_b<V, RowT, SG>), dim3(g), dim3(256), 0, st, B->rows, (const RowT *)B->row_ptr,  \
+                     B->col, (const RowT *)AT->row_ptr, AT->col, (const V *)AT->val, (const V *)A->val, at_perm, \
+                     (const RowT *)C->row_ptr, C->col, dc, db)
+    if (avg <= 4) GRAD_B(4);
+    else if (avg <= 8) GRAD_B(8);
+    else if (avg <= 16) GRAD_B(16);
+    else if (avg <= 32) GRAD_B(32);
+    else GRAD_B(64);
+#undef GRAD_B
   }
   NSP_HIP(h, hipGetLastError());
   uint32_t tl = 0;
   NSP_HIP(h, hipMemcpyAsync(&tl, too_long, 4, hipMemcpyDeviceToHost, st));
   NSP_HIP(h, hipStreamSynchronize(st));
-  if (tl) return set_error(h, NSP_ERR_UNSUPPORTED, "backward: a C row exceeds the dA table");
+  if (tl) return set_error(h, NSP_ERR_UNSUPPORTED, "backward: C rows with more than 1024 entries");
   return NSP_OK;
 }
 
 int spgemm_backward(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
-                    const nsp_csr_t *AT, const nsp_csr_t *C, const void *dc, void *da, void *db) {
+                    const nsp_csr_t *AT, const void *at_perm, const nsp_csr_t *C, const void *dc, void *da,
+                    void *db) {
   if (!A || !B || !AT || !C || !A->row_ptr || !B->row_ptr || !AT->row_ptr || !C->row_ptr ||
-      A->cols != B->rows || AT->rows != A->cols || C->rows != A->rows)
+      A->cols != B->rows || AT->rows != A->cols || AT->nnz != A->nnz || C->rows != A->rows)
     return set_error(h, NSP_ERR_INVALID, "nsp_spgemm_backward: bad arguments");
   if ((A->nnz && !da) || (B->nnz && !db) || (C->nnz && !dc))
     return set_error(h, NSP_ERR_INVALID, "nsp_spgemm_backward: NULL gradient buffer");
+  if (AT->nnz && (!AT->col || (!AT->val && !at_perm)))
+    return set_error(h, NSP_ERR_INVALID, "nsp_spgemm_backward: A^T needs its columns and values or a permutation");
   if (it == NSP_IDX32 && vt == NSP_F64)
-    return backward_impl<double, int32_t>(h, A, B, AT, C, (const double *)dc, (double *)da, (double *)db);
-  if (it == NSP_IDX32) return backward_impl<float, int32_t>(h, A, B, AT, C, (const float *)dc, (float *)da, (float *)db);
+    return backward_impl<double, int32_t>(h, A, B, AT, (const int32_t *)at_perm, C, (const double *)dc,
+                                          (double *)da, (double *)db);
+  if (it == NSP_IDX32)
+    return backward_impl<float, int32_t>(h, A, B, AT, (const int32_t *)at_perm, C, (const float *)dc, (float *)da,
+                                         (float *)db);
   if (vt == NSP_F64)
-    return backward_impl<double, int64_t>(h, A, B, AT, C, (const double *)dc, (double *)da, (double *)db);
-  return backward_impl<float, int64_t>(h, A, B, AT, C, (const float *)dc, (float *)da, (float *)db);
+    return backward_impl<double, int64_t>(h, A, B, AT, (const int64_t *)at_perm, C, (const double *)dc,
+                                          (double *)da, (double *)db);
+  return backward_impl<float, int64_t>(h, A, B, AT, (const int64_t *)at_perm, C, (const float *)dc, (float *)da,
+                                       (float *)db);
 }
 
 }  // namespace nsp

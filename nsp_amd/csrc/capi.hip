@@ -156,20 +156,20 @@ int nsp_get_stats(nsp_handle_t hs, nsp_stats_t *out) {
 int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
   if (!hs || !name) return NSP_ERR_INVALID;
   Handle *h = &hs->h;
-  if (!std::strcmp(name, "sub_lanes")) {
-    if (value != 0 && value != 1 && value != 4 && value != 16)
-      return nsp::set_error(h, NSP_ERR_INVALID, "sub_lanes must be 0, 1, 4 or 16");
-    h->sub_lanes = (int)value;
-  } else if (!std::strcmp(name, "sym_factor")) {
-    if (value != 1 && value != 2) return nsp::set_error(h, NSP_ERR_INVALID, "sym_factor must be 1 or 2");
-    h->sym_factor = (int)value;
-  } else if (!std::strcmp(name, "sym_rows_per_wave")) {
+  auto rpw = [&](int *dst, const char *what) {
     if (value != 1 && value != 2 && value != 4)
-      return nsp::set_error(h, NSP_ERR_INVALID, "sym_rows_per_wave must be 1, 2 or 4");
-    h->sym_rows_per_wave = (int)value;
+      return nsp::set_error(h, NSP_ERR_INVALID, std::string(what) + " must be 1, 2 or 4");
+    *dst = (int)value;
+    return (int)NSP_OK;
+  };
+  if (!std::strcmp(name, "sym_small_rows_per_wave")) {
+    return rpw(&h->sym_small_rows_per_wave, name);
+  } else if (!std::strcmp(name, "sym_rows_per_wave")) {
+    return rpw(&h->sym_rows_per_wave, name);
+  } else if (!std::strcmp(name, "num_small_rows_per_wave")) {
+    return rpw(&h->num_small_rows_per_wave, name);
   } else if (!std::strcmp(name, "num_rows_per_wave")) {
-    if (value != 1 && value != 2) return nsp::set_error(h, NSP_ERR_INVALID, "num_rows_per_wave must be 1 or 2");
-    h->num_rows_per_wave = (int)value;
+    return rpw(&h->num_rows_per_wave, name);
   } else if (!std::strcmp(name, "num_table_factor")) {
     if (value != 2 && value != 4) return nsp::set_error(h, NSP_ERR_INVALID, "num_table_factor must be 2 or 4");
     h->num_table_factor = (int)value;
@@ -328,9 +328,10 @@ int nsp_transpose(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr
 }
 
 int nsp_spgemm_backward(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
-                        const nsp_csr_t *AT, const nsp_csr_t *C, const void *dC, void *dA, void *dB) {
+                        const nsp_csr_t *AT, const void *at_perm, const nsp_csr_t *C, const void *dC, void *dA,
+                        void *dB) {
   if (!hs) return NSP_ERR_INVALID;
-  return nsp::spgemm_backward(&hs->h, it, vt, A, B, AT, C, dC, dA, dB);
+  return nsp::spgemm_backward(&hs->h, it, vt, A, B, AT, at_perm, C, dC, dA, dB);
 }
 
 int nsp_gather_rows_fill(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *B, int64_t nr,

@@ -60,10 +60,10 @@ struct Handle {
   nsp_stats_t stats{};
 
   // tuning knobs
-  int sub_lanes = 0;
-  int sym_factor = 1;
   int grid_cap = 8;
+  int sym_small_rows_per_wave = 4;
   int sym_rows_per_wave = 2;
+  int num_small_rows_per_wave = 4;
   int num_rows_per_wave = 2;
   int num_table_factor = 2;
   bool timing = false;

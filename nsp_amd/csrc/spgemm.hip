@@ -57,12 +57,21 @@ __global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles, int64_
 // per-bin histogram, nprod, an int64 copy of A's row pointer, and for every
 // A entry (i,k) the start and length of B row k (ab_b0, ab_len), which the
 // hash passes then read with one independent load instead of the dependent
-// A.col -> B.row_ptr gather.  16 lanes per row so the A.col reads coalesce.
+// A.col -> B.row_ptr gather.
+//
+// Entry-parallel: a block takes a chunk of kInfoRows consecutive rows, stages
+// their row pointers in LDS, and walks the chunk's entries one per thread
+// (coalesced A.col reads and ab_* writes, whatever the row lengths); each
+// entry finds its row by binary search over the staged pointers and adds its
+// B row length to the row's LDS counter.  One thread per row then finishes
+// the row.
 //
 // Batched products (a_row_off != NULL): A and B hold `count` matrices as
 // concatenated rows with matrix-local columns; row i of matrix b selects B
 // row b_row_off[b] + k for its column k.  This is the only place the batch
 // structure enters: every later pass works on the concatenated rows.
+constexpr int kInfoRows = 256;
+
 template <typename RowT>
 __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
                                                  const int32_t *__restrict__ acol,
@@ -75,63 +84,89 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
                                                  int64_t *__restrict__ intprod,
                                                  int32_t *__restrict__ row_nz,
                                                  uint8_t *__restrict__ cls, Info *info) {
-  constexpr int GS = 16;
+  __shared__ int64_t s_rpt[kInfoRows + 1];
+  __shared__ int64_t s_base[kInfoRows];
+  __shared__ unsigned long long s_ip[kInfoRows];
   __shared__ unsigned long long hist[kBins];
   __shared__ unsigned long long s_nprod, s_max;
-  if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
-  if (threadIdx.x == 0) { s_nprod = 0; s_max = 0; }
-  __syncthreads();
-  const int glane = threadIdx.x % GS;
+  __shared__ int64_t s_mat[2];
+  const int t = threadIdx.x;
+  if (t < kBins) hist[t] = 0;
+  if (t == 0) { s_nprod = 0; s_max = 0; }
   unsigned long long my_sum = 0, my_max = 0;
-  for (int64_t i = blockIdx.x * (int64_t)(256 / GS) + threadIdx.x / GS; i < MA;
-       i += (int64_t)gridDim.x * (256 / GS)) {
-    const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
-    int64_t bbase = 0;
-    if (a_row_off) {
-      // matrix of row i: last b with a_row_off[b] <= i
+  for (int64_t r0 = blockIdx.x * (int64_t)kInfoRows; r0 < MA; r0 += (int64_t)gridDim.x * kInfoRows) {
+    const int nr = (int)(MA - r0 < kInfoRows ? MA - r0 : kInfoRows);
+    for (int q = t; q <= nr; q += 256) s_rpt[q] = (int64_t)arpt[r0 + q];
+    if (t < nr) s_ip[t] = 0;
+    if (a_row_off && t < 2) {
+      // matrices of the chunk's first and last rows: last b with a_row_off[b] <= i
+      const int64_t i = r0 + (t == 0 ? 0 : nr - 1);
       int64_t lo = 0, hi = count;
       while (hi - lo > 1) {
         const int64_t mid = (lo + hi) >> 1;
         if (a_row_off[mid] <= i) lo = mid; else hi = mid;
       }
-      bbase = b_row_off[lo];
+      s_mat[t] = lo;
     }
-    int64_t s = 0;
-    for (int64_t p = p0 + glane; p < p1; p += GS) {
-      const int64_t k = bbase + acol[p];
+    __syncthreads();
+    if (t < nr) {
+      int64_t bb = 0;
+      if (a_row_off) {
+        const int64_t i = r0 + t;
+        int64_t lo = s_mat[0], hi = s_mat[1] + 1;
+        while (hi - lo > 1) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (a_row_off[mid] <= i) lo = mid; else hi = mid;
+        }
+        bb = b_row_off[lo];
+      }
+      s_base[t] = bb;
+    }
+    __syncthreads();
+    const int64_t e0 = s_rpt[0], e1 = s_rpt[nr];
+    for (int64_t p = e0 + t; p < e1; p += 256) {
+      // row of entry p: the last q with s_rpt[q] <= p (empty rows skipped)
+      int lo = 0, hi = nr;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_rpt[mid] <= p) lo = mid; else hi = mid;
+      }
+      const int64_t k = s_base[lo] + acol[p];
       const int64_t b0 = (int64_t)brpt[k];
       const int64_t len = (int64_t)brpt[k + 1] - b0;
       ab_b0[p] = b0;
       ab_len[p] = (int32_t)len;
-      s += len;
+      atomicAdd(&s_ip[lo], (unsigned long long)len);
     }
-    s = group_sum<GS>(s);
-    if (glane == 0) {
-      intprod[i] = s;
-      arpt64[i] = p0;
+    __syncthreads();
+    if (t < nr) {
+      const int64_t i = r0 + t;
+      const unsigned long long s = s_ip[t];
+      intprod[i] = (int64_t)s;
+      arpt64[i] = s_rpt[t];
       if (s == 0) {
         cls[i] = kNoBin;
         row_nz[i] = 0;
       } else {
-        const int b = sym_bin_of(s);
+        const int b = sym_bin_of((int64_t)s);
         cls[i] = (uint8_t)b;
         atomicAdd(&hist[b], 1ull);
       }
-      my_sum += (unsigned long long)s;
-      my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
+      my_sum += s;
+      my_max = my_max > s ? my_max : s;
     }
+    __syncthreads();  // the next chunk reuses s_rpt, s_ip, s_mat
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) arpt64[MA] = (int64_t)arpt[MA];
+  if (blockIdx.x == 0 && t == 0) arpt64[MA] = (int64_t)arpt[MA];
   my_sum = group_sum<64>(my_sum);
   my_max = group_max<64>(my_max);
-  if ((threadIdx.x & 63) == 0) {
+  if ((t & 63) == 0) {
     atomicAdd(&s_nprod, my_sum);
     atomicMax(&s_max, my_max);
   }
   __syncthreads();
-  if (threadIdx.x < kBins && hist[threadIdx.x])
-    atomicAdd((unsigned long long *)&info->count_sym[threadIdx.x], hist[threadIdx.x]);
-  if (threadIdx.x == 0) {
+  if (t < kBins && hist[t]) atomicAdd((unsigned long long *)&info->count_sym[t], hist[t]);
+  if (t == 0) {
     atomicAdd((unsigned long long *)&info->nprod, s_nprod);
     atomicMax((unsigned long long *)&info->max_intprod, s_max);
   }
@@ -547,26 +582,24 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
 #define SYM_BLOCK(BS, SH)                                                                          \
   hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, cap)), dim3(BS), 0, \
                      st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
-  // bins 3..5: 1, 2 or 4 rows per wavefront (more rows in flight per CU
-  // against fewer lanes per row); tuned by "sym_rows_per_wave"
-  const int rpw = h->sym_rows_per_wave;
+  // wave-level bins: 1, 2 or 4 rows per wavefront (GS = 64, 32, 16 lanes per
+  // row; more rows in flight per CU against fewer lanes per row), set by
+  // "sym_small_rows_per_wave" for bins 0..2 and "sym_rows_per_wave" for 3..5
+  const int rpw = bin <= 2 ? h->sym_small_rows_per_wave : h->sym_rows_per_wave;
+#define SYM_RPW(SH) \
+  if (rpw >= 4) SYM_GROUP(16, SH); else if (rpw == 2) SYM_GROUP(32, SH); else SYM_GROUP(64, SH);
   switch (bin) {
-    case 0: SYM_GROUP(16, 32); break;
-    case 1: SYM_GROUP(32, 64); break;
-    case 2: SYM_GROUP(64, 128); break;
-    case 3:
-      if (rpw >= 4) SYM_GROUP(16, 256); else if (rpw == 2) SYM_GROUP(32, 256); else SYM_GROUP(64, 256);
-      break;
-    case 4:
-      if (rpw >= 4) SYM_GROUP(16, 512); else if (rpw == 2) SYM_GROUP(32, 512); else SYM_GROUP(64, 512);
-      break;
-    case 5:
-      if (rpw >= 4) SYM_GROUP(16, 1024); else if (rpw == 2) SYM_GROUP(32, 1024); else SYM_GROUP(64, 1024);
-      break;
+    case 0: SYM_RPW(32); break;
+    case 1: SYM_RPW(64); break;
+    case 2: SYM_RPW(128); break;
+    case 3: SYM_RPW(256); break;
+    case 4: SYM_RPW(512); break;
+    case 5: SYM_RPW(1024); break;
     case 6: SYM_BLOCK(256, 2048); break;
     case 7: SYM_BLOCK(512, 16384); break;
     default: break;
   }
+#undef SYM_RPW
 #undef SYM_GROUP
 #undef SYM_BLOCK
 }
@@ -589,26 +622,28 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
   hipLaunchKernelGGL((k_num_block<V, BS, NS, SHF, kNumR>), dim3(grid_for(h, count, 1, cap)), dim3(BS),   \
                      0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval,     \
                      crpt, ccol, cval)
-  // bins 3..4: 1 or 2 rows per wavefront ("num_rows_per_wave"); table
-  // factor 2 or 4 slots per row entry ("num_table_factor") for bins 0..4
-  const int rpw = h->num_rows_per_wave;
+  // wave-level bins: 1, 2 or 4 rows per wavefront (GS = 64, 32, 16; never
+  // more lanes than the bin's largest row has entries), set by
+  // "num_small_rows_per_wave" for bins 0..2 and "num_rows_per_wave" for 3..4;
+  // table factor 2 or 4 slots per row entry ("num_table_factor") for 0..3
+  const int rpw = bin <= 2 ? h->num_small_rows_per_wave : h->num_rows_per_wave;
   const bool f4 = h->num_table_factor >= 4;
+#define NUM_F(GS, NS) \
+  if (f4) NUM_GROUP(GS, NS, 4); else NUM_GROUP(GS, NS, 2);
   switch (bin) {
-    case 0: if (f4) NUM_GROUP(16, 16, 4); else NUM_GROUP(16, 16, 2); break;
-    case 1: if (f4) NUM_GROUP(32, 32, 4); else NUM_GROUP(32, 32, 2); break;
-    case 2: if (f4) NUM_GROUP(64, 64, 4); else NUM_GROUP(64, 64, 2); break;
-    case 3:
-      if (rpw >= 2) { if (f4) NUM_GROUP(32, 128, 4); else NUM_GROUP(32, 128, 2); }
-      else { if (f4) NUM_GROUP(64, 128, 4); else NUM_GROUP(64, 128, 2); }
-      break;
+    case 0: NUM_F(16, 16); break;
+    case 1: if (rpw >= 4) { NUM_F(16, 32); } else { NUM_F(32, 32); } break;
+    case 2: if (rpw >= 4) { NUM_F(16, 64); } else if (rpw == 2) { NUM_F(32, 64); } else { NUM_F(64, 64); } break;
+    case 3: if (rpw >= 4) { NUM_F(16, 128); } else if (rpw == 2) { NUM_F(32, 128); } else { NUM_F(64, 128); } break;
     case 4:
-      if (rpw >= 2) NUM_GROUP(32, 256, 2); else NUM_GROUP(64, 256, 2);
+      if (rpw >= 4) NUM_GROUP(16, 256, 2); else if (rpw == 2) NUM_GROUP(32, 256, 2); else NUM_GROUP(64, 256, 2);
       break;
     case 5: NUM_GROUP(64, 512, 2); break;
     case 6: NUM_BLOCK(256, 2048, 2); break;
     case 7: NUM_BLOCK(512, 4096, 2); break;
     default: break;
   }
+#undef NUM_F
 #undef NUM_GROUP
 #undef NUM_BLOCK
 }
@@ -744,7 +779,8 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
   timing_mark(h, 0);
   hipLaunchKernelGGL(k_init, dim3(std::max<int64_t>(1, std::min<int64_t>((h->ntiles + 255) / 256, 1024))),
                      dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles, cap);
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 15) / 16, (int64_t)h->num_cu * 8));
+  const int g =
+      (int)std::max<int64_t>(1, std::min<int64_t>((MA + kInfoRows - 1) / kInfoRows, (int64_t)h->num_cu * 8));
   hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr,
                      A->col, (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->a_rpt64, h->ab_b0,
                      h->ab_len, h->intprod, h->row_nz, h->cls, h->d_info);
@@ -774,6 +810,7 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
     for (int q = 0; q < n; q++)
       launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, h->ab_len, B->col);
     h->stats.sym_launches = n;
+    if (n == 1 && L[0].direct >= 0) h->stats.direct_mode |= 1;
   }
   timing_mark(h, 2);
   NSP_HIP(h, hipGetLastError());
@@ -797,7 +834,6 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
   h->stats.max_row_nnz = hi.max_nz;
   h->stats.n_sym_bins = kBins;
   h->stats.n_num_bins = kBins;
-  h->stats.sub_lanes = 0;
   for (int b = 0; b < kBins; b++) {
     h->stats.sym_bin[b] = hi.count_sym[b];
     h->stats.num_bin[b] = hi.count_num[b];
@@ -841,6 +877,7 @@ static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
     launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, (const V *)A->val,
                       h->ab_len, B->col, (const V *)B->val, C->col, (V *)C->val);
   h->stats.num_launches = n;
+  h->stats.direct_mode = (h->stats.direct_mode & 1) | ((n == 1 && L[0].direct >= 0) ? 2 : 0);
   timing_mark(h, 5);
   h->timing_pending = h->timing;
   NSP_HIP(h, hipGetLastError());

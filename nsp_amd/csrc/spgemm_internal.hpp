@@ -25,7 +25,8 @@ int graph_destroy(GraphPlan *gp);
 int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B);
 int transpose(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp_csr_t *XT);
 int spgemm_backward(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
-                    const nsp_csr_t *AT, const nsp_csr_t *C, const void *dc, void *da, void *db);
+                    const nsp_csr_t *AT, const void *at_perm, const nsp_csr_t *C, const void *dc, void *da,
+                    void *db);
 int shrink_cols(Handle *h, int64_t nnz, const int32_t *col, int64_t l, int64_t lo, int64_t hi,
                 int32_t *new_col, int32_t *perm, int64_t perm_cap, int64_t *shrinked_m);
 int gather_rows_count(Handle *h, nsp_index_t it, const nsp_csr_t *B, int64_t nr, const int32_t *rows,

@@ -194,22 +194,27 @@ def uneven_batch_concat(count: int = 4096, seed: int = 0, nmin: int = 64, nmax: 
     M = int(row_off[-1])
     nrow = np.repeat(ns, ns)                          # n of each row's matrix
     k = np.minimum(np.maximum(rng.poisson(lam, M), 1), nrow)
+    rpt = np.zeros(M + 1, np.int64)
+    np.cumsum(k, out=rpt[1:])
+    S = np.int64(nmax + 1)
     rows = np.repeat(np.arange(M, dtype=np.int64), k)
-    nn = np.repeat(nrow, k)
-    cols = (rng.random(rows.size) * nn).astype(np.int64)
-    # redraw duplicates within a row until every row's columns are distinct
+    key = rows * S + (rng.random(rows.size) * np.repeat(nrow, k)).astype(np.int64)
+    del rows
+    key.sort()
+    # a duplicate (row, col) is dropped and its row draws again; the sorted
+    # distinct keys and the sorted redraws are merged (two runs: linear)
     while True:
-        key = rows * (nmax + 1) + cols
-        order = np.argsort(key, kind="stable")
-        ks = key[order]
-        dup = np.zeros(rows.size, bool)
-        dup[order[1:]] = ks[1:] == ks[:-1]
-        nd = int(dup.sum())
-        if nd == 0:
+        dup = np.flatnonzero(key[1:] == key[:-1]) + 1
+        if dup.size == 0:
             break
-        cols[dup] = (rng.random(nd) * nn[dup]).astype(np.int64)
-    vals = rng.uniform(-1.0, 1.0, rows.size)
-    X = _from_rows_cols(M, int(ns.max()) if count else 0, rows, cols, vals)
+        r = key[dup] // S
+        redraw = r * S + (rng.random(dup.size) * nrow[r]).astype(np.int64)
+        redraw.sort()
+        key = np.sort(np.concatenate([np.delete(key, dup), redraw]), kind="stable")
+    cols = (key % S).astype(np.int32)
+    del key
+    vals = rng.uniform(-1.0, 1.0, cols.size)
+    X = HostCSR(rpt, cols, vals, M, int(ns.max()) if count else 0)
     return X, row_off
 
 

@@ -134,3 +134,58 @@ def test_autograd_finite_differences(nsp, orc):
         worst = max(worst, abs(fd - gb[e]) / max(1.0, abs(gb[e])))
     print(f"  worst FD relative error {worst:.2e}", flush=True)
     assert worst <= 1e-6
+
+
+def _mixed(nrows, ncols, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    lens = np.clip(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), nrows)).astype(np.int64), 0, ncols)
+    rows = np.repeat(np.arange(nrows), lens)
+    cols = np.concatenate([rng.choice(ncols, int(k), replace=False) for k in lens])
+    return gen._from_rows_cols(nrows, ncols, rows, cols, rng.uniform(-1, 1, rows.size))
+
+
+@pytest.mark.parametrize("case", ["stencil", "mixed"])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("use_perm", [False, True])
+def test_backward_general_patterns(nsp, orc, case, dtype, use_perm):
+    """A*A on the 27-point stencil (C rows up to 125 entries) and a product of
+    mixed row lengths (C rows past 128, the second table tier), with A's
+    values given as A^T values or through the A^T -> A permutation."""
+    print("case: backward general", case, dtype, use_perm, flush=True)
+    import torch
+    if case == "stencil":
+        A = B = to_dtype(gen.stencil_27pt(10, seed=4), dtype)
+    else:
+        A = to_dtype(_mixed(600, 500, 1, 40, seed=5), dtype)
+        B = to_dtype(_mixed(500, 700, 1, 30, seed=6), dtype)
+    h = nsp.Handle()
+    dA_ = nsp.DeviceCSR.from_host(A, dtype)
+    dB_ = nsp.DeviceCSR.from_host(B, dtype)
+    C = h.spgemm(dA_, dB_)
+    crpt = C.row_ptr.cpu().numpy().astype(np.int64)
+    if case == "mixed":
+        assert np.diff(crpt).max() > 128
+    if use_perm:
+        idx = torch.arange(A.nnz, dtype=torch.float64, device="cuda")
+        ATi = h.transpose(nsp.DeviceCSR(dA_.row_ptr, dA_.col, idx, A.nrows, A.ncols))
+        perm = ATi.val.to(dA_.row_ptr.dtype)
+        AT = nsp.DeviceCSR(ATi.row_ptr, ATi.col, None, ATi.nrows, ATi.ncols)
+    else:
+        perm, AT = None, h.transpose(dA_)
+    rng = np.random.default_rng(8)
+    dc = rng.standard_normal(C.nnz)
+    if dtype == "f32":
+        dc = dc.astype(np.float32).astype(np.float64)
+    da, db = h.backward(dA_, dB_, AT, C, torch.from_numpy(dc).to(device="cuda", dtype=C.val.dtype),
+                        at_perm=perm)
+    ccol = C.col.cpu().numpy()
+    ref_da, ref_db = orc.backward(A, B, crpt, ccol, dc)
+    abs_da, abs_db = orc.backward(abs_csr(A), abs_csr(B), crpt, ccol, np.abs(dc))
+    err_a = np.max(np.abs(da.cpu().numpy() - ref_da) / np.where(abs_da > 0, abs_da, 1))
+    err_b = np.max(np.abs(db.cpu().numpy() - ref_db) / np.where(abs_db > 0, abs_db, 1))
+    print(f"  componentwise error dA {err_a:.2e} dB {err_b:.2e}", flush=True)
+    assert err_a <= TOL[dtype] and err_b <= TOL[dtype]
+    # deterministic: a second call gives the same bits
+    da2, db2 = h.backward(dA_, dB_, AT, C, torch.from_numpy(dc).to(device="cuda", dtype=C.val.dtype),
+                          at_perm=perm)
+    assert torch.equal(da, da2) and torch.equal(db, db2)

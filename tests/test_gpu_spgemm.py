@@ -77,15 +77,25 @@ def test_all_bins(nsp, orc, dtype):
     check_against_oracle(A, B, C, dtype, orc)
 
 
-@pytest.mark.parametrize("sub", [1, 4, 16])
-def test_sub_lane_variants(nsp, orc, sub):
-    print("case: sub lanes", sub, flush=True)
-    A = mixed_rows(800, 700, 1, 60, seed=3)
-    B = mixed_rows(700, 900, 1, 60, seed=4)
-    h = nsp.Handle()
-    h.set_param("sub_lanes", sub)
-    C, _ = run(nsp, A, B, "f64", handle=h)
-    check_against_oracle(A, B, C, "f64", orc)
+@pytest.mark.parametrize("rpw", [1, 2, 4])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_rows_per_wave_and_table_variants(nsp, orc, rpw, dtype):
+    """Every wave-level kernel shape (1, 2 or 4 rows per wavefront in each
+    bin group, table factor 2 and 4) gives the same product."""
+    print("case: rows per wave", rpw, dtype, flush=True)
+    A = to_dtype(mixed_rows(1500, 1200, 1, 60, seed=3), dtype)
+    B = to_dtype(mixed_rows(1200, 1600, 1, 40, seed=4), dtype)
+    for tf in (2, 4):
+        h = nsp.Handle()
+        for name in ("sym_small_rows_per_wave", "sym_rows_per_wave", "num_small_rows_per_wave",
+                     "num_rows_per_wave"):
+            h.set_param(name, rpw)
+        h.set_param("num_table_factor", tf)
+        C, _ = run(nsp, A, B, dtype, handle=h)
+        st = h.stats()
+        assert all(c > 0 for c in st["sym_bin"][:6]), st["sym_bin"]
+        assert all(c > 0 for c in st["num_bin"][:5]), st["num_bin"]
+        check_against_oracle(A, B, C, dtype, orc)
 
 
 def test_structural_zeros_and_empty_rows(nsp, orc):
