@@ -204,6 +204,38 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
             "structure_equal": bool(same), "max_componentwise_diff": err}
 
 
+def rmat_leg(scale: int = 20, steps: int = 3, warmup: int = 1):
+    """R-MAT A*A in FP64 with int64 row pointers (power-law rows; the heavy
+    rows take the LDS-bitmap path).  C's buffers are reused across steps."""
+    import torch
+    import nsp_amd
+    from nsp_amd import gen
+    A = gen.rmat(scale, seed=12345)
+    dA = nsp_amd.DeviceCSR.from_host(A, "f64", "i64")
+    h = nsp_amd.Handle()
+    h.set_param("timing", 1)
+    rpt, nnz = h.symbolic(dA, dA)
+    C = h.numeric(dA, dA, rpt, nnz)
+    st = h.stats()
+
+    def step():
+        r, z = h.symbolic(dA, dA)
+        h.numeric(dA, dA, r, z, out=nsp_amd.DeviceCSR(r, C.col, C.val, C.nrows, C.ncols))
+
+    ms = _timed(step, steps, warmup)
+    st2 = h.stats()
+    out = {"workload": f"R-MAT scale {scale} (a,b,c)=(.57,.19,.19), 16 edges/vertex, A*A FP64, int64 row_ptr",
+           "rows": A.nrows, "nnz_A": A.nnz, "nnz_C": nnz, "nprod": st["nprod"],
+           "ms_per_step": round(ms, 3), "GFLOPs": round(2.0 * st["nprod"] / ms / 1e6, 1),
+           "phases_ms": {"binning": round(st2["t_bin_ms"], 3), "symbolic": round(st2["t_sym_ms"], 3),
+                         "scan": round(st2["t_scan_ms"], 3), "numeric": round(st2["t_num_ms"], 3)},
+           "heavy_rows": {"symbolic": st["sym_bin"][8], "numeric": st["num_bin"][8]},
+           "max_intprod": st["max_intprod"], "max_row_nnz": st["max_row_nnz"]}
+    del C, rpt
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -211,8 +243,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="fwd_bwd,batched",
-                    help="extra measurements after the headline: fwd_bwd, batched (comma list, or none)")
+    ap.add_argument("--legs", default="fwd_bwd,batched,rmat",
+                    help="extra measurements after the headline: fwd_bwd, batched, rmat "
+                         "(comma list, or none)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--param", action="append", default=[],
                     help="library tuning parameter name=value (repeatable)")
@@ -298,6 +331,9 @@ def main():
         out["fwd_bwd"] = {dt: fwd_bwd_leg(dt) for dt in ("f64", "f32")}
     if "batched" in legs:
         out["batched"] = batched_leg(args.batch_count)
+        torch.cuda.empty_cache()
+    if "rmat" in legs:
+        out["rmat"] = rmat_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, B, nprod)
     print(json.dumps(out), flush=True)

@@ -97,6 +97,8 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *   "num_rows_per_wave"  rows per wavefront in the numeric bins of 65..256
  *                  entries (1, 2, 4)
  *   "num_table_factor"   numeric hash slots per row entry (2, 4)
+ *   "heavy_blocks" blocks in flight for rows past 16384 products / 4096
+ *                  entries (0 = one per CU)
  *   "timing"       1 records HIP events around each phase (see nsp_stats_t) */
 int nsp_set_param(nsp_handle_t h, const char *name, int64_t value);
 

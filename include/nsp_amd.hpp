@@ -1,0 +1,154 @@
+/* C++ host API of nsp_amd: the reference's types and entry points for the
+ * local sparse product, on one device, over the C ABI of nsp_amd.h.
+ *
+ * Names, fields and argument meaning follow the reference (amavliut/nsp):
+ *   sfCSR, nsp_spgemm_kernel_hash     nsp/inc/nsp.h:57-70, 105
+ *   csrlocinfo                        src/csrseg.h:3-9
+ *   CSR (fields the product uses)     src/CSR.h:67-104
+ *   CSRm::init/free/copyToDevice/...  src/CSR.cu:28, 572-621, 656-700
+ *   shrink_col, get_shrinked_matrix   src/spspmpi.cu:106-182
+ *   nsparseMGPU_version0              src/spspmpi.cu:266-313
+ *   nsparseMGPU_version1              src/spspmpi.cu:371-975, without MPI:
+ *                                     the halo rows of P are gathered from the
+ *                                     full P on the same device
+ *   read_binary_matrix                src/READ_MATRIX_MPI.cpp:146-356 with the
+ *                                     row split and column shift of
+ *                                     src/distribute.cu:1071-1125
+ *   write_binary_matrix               main.cu:468-512
+ * Errors: where the reference exits (checkCudaErrors, helper_cuda.h:714) these
+ * functions throw nspamd::Error carrying the C ABI status.  Calls are
+ * host-synchronous on return, as in the reference; each host thread uses its
+ * own library handle on the current device.
+ *
+ * Link with -lnsp_amd_cpp -lnsp_amd (and the HIP runtime). */
+#ifndef NSP_AMD_HPP
+#define NSP_AMD_HPP
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "nsp_amd.h"
+
+namespace nspamd {
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string &msg) : std::runtime_error(msg), status(s) {}
+};
+
+/* nsp/inc/nsp.h:57-70.  The product reads M, N, d_rpt, d_col, d_val of a and
+ * b and sets M, N, nnz and the device arrays of c. */
+typedef struct {
+  int *rpt;
+  int *col;
+  double *val;
+  int *d_rpt;
+  int *d_col;
+  double *d_val;
+  unsigned int M;
+  unsigned int N;
+  int nnz;
+  int nnz_max;
+  char *matrix_name;
+} sfCSR;
+
+/* C = A*B (nsp/src/nsp.cu:2448-2488): sorted rows, structural zeros kept;
+ * c->d_rpt, c->d_col, c->d_val are allocated here with hipMalloc (the caller
+ * frees them, e.g. with sfCSR_free_device). */
+void nsp_spgemm_kernel_hash(sfCSR *a, sfCSR *b, sfCSR *c);
+void sfCSR_free_device(sfCSR *c);
+
+/* src/csrseg.h:3-9 (the rank's slice of P; kept for the signature of
+ * nsparseMGPU_version0, which does not read it). */
+typedef struct {
+  int fr;
+  int lr;
+  int *row;
+  int *col;
+  double *val;
+} csrlocinfo;
+
+/* The CSR fields of src/CSR.h:67-104 that the local product and its
+ * compaction read or write.  Arrays are device pointers when on_the_device,
+ * host pointers otherwise.  Columns of a rank's block are in the rank frame
+ * (global column - row_shift). */
+struct CSR {
+  int nnz = 0;                 // entries
+  int n = 0;                   // rows
+  int64_t m = 0;               // columns
+  int shrinked_m = 0;          // columns after compaction
+  int64_t full_n = 0;          // rows of the whole matrix
+  int64_t full_m = 0;          // columns of the whole matrix
+  bool on_the_device = false;
+  bool shrinked_flag = false;
+  int64_t row_shift = 0;       // first global row of the block
+  double *val = nullptr;
+  int *col = nullptr;
+  int *row = nullptr;          // n+1 row pointers
+  int *shrinked_col = nullptr; // compacted columns (device)
+  /* the compaction map of shrink_col: shrinking_permut[c'] = rank-frame
+   * column of compacted column c' (the reference keeps it in a file-static
+   * buffer, src/spspmpi.cu:136-139) */
+  int *shrinking_permut = nullptr;
+  int shrinking_permut_len = 0;
+  /* bounds of the rank-frame columns [col_lo, col_hi) used by shrink_col;
+   * default [-row_shift, m - row_shift) */
+  int64_t col_lo = 0, col_hi = -1;
+};
+
+namespace CSRm {
+CSR *init(int n, int64_t m, int nnz, bool allocate, bool on_the_device, int64_t full_n = -1,
+          int64_t row_shift = 0);
+void free(CSR *A);
+/* frees only the struct, not its arrays (src/CSR.cu:624): for views such as
+ * get_shrinked_matrix's result */
+void freeStruct(CSR *A);
+CSR *copyToDevice(const CSR *A);
+CSR *copyToHost(const CSR *A);
+/* ||A||_F of a host or device matrix (main.cu:236-241 prints it per rank) */
+double norm_fro(const CSR *A);
+}  // namespace CSRm
+
+/* Compacts A's columns once (src/spspmpi.cu:141-162): perm = sorted
+ * unique(cols U [0..l]) with l = P->n - 1 when P is given, else A->n - 1;
+ * A->shrinked_col, A->shrinked_m and A->shrinking_permut are set.  Returns
+ * false when A was already compacted. */
+bool shrink_col(CSR *A, CSR *P);
+/* A view of A with compacted columns (src/spspmpi.cu:166-182); shares A's
+ * row and val arrays. */
+CSR *get_shrinked_matrix(CSR *A, CSR *P);
+
+/* C = A*P on the device (src/spspmpi.cu:266-313): C->n = A->n,
+ * C->m = P->m, C->full_n = A->full_n, C->row_shift = A->row_shift.
+ * loc_time gets the product's time in seconds (host-synchronous around the
+ * call, as the reference's timer). */
+CSR *nsparseMGPU_version0(CSR *A, CSR *P, csrlocinfo *Plocal, double &loc_time);
+CSR *nsparseMGPU_version0(CSR *A, CSR *P);
+
+/* One rank's local product on one device (src/spspmpi.cu:371-975): Alocal is
+ * the rank's row block (device, rank-frame columns, row_shift = first row);
+ * Pfull holds every row of P (device, global columns).  completedP = P rows
+ * shrinking_permut(A) + row_shift in ascending order, columns re-shifted to
+ * the rank frame; then A_ and P_ are compacted and C = A_ * P_ (C's columns
+ * are P_'s compacted columns).  p_local_n: rows of P the rank owns (default
+ * Alocal->n).  A_out / P_out, when given, receive A_ and P_: A_ is a view of
+ * Alocal's arrays (release it with CSRm::freeStruct), P_ owns its arrays
+ * (CSRm::free). */
+CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local_n = -1, CSR **A_out = nullptr,
+                          CSR **P_out = nullptr);
+
+/* Rank `rank` of `nprocs` of a binary COO matrix file (header {nr, nc, nt}
+ * when header_type = 1, {nr, nt} when 0, then nt records {int64 i, int64 j,
+ * double v}, 1-based, sorted by row): host CSR with n = rows present in the
+ * rank's range, columns shifted by the block's first row (row_shift), rows
+ * sorted by column.  *nr_out/*nc_out/*nt_out get the file's sizes. */
+CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type = 1, int64_t *nr_out = nullptr,
+                        int64_t *nc_out = nullptr, int64_t *nt_out = nullptr);
+/* Writes a host CSR as binary COO (header_type 1); global column =
+ * col + col_shift. */
+void write_binary_matrix(const char *path, const CSR *A, int64_t col_shift = 0);
+
+}  // namespace nspamd
+
+#endif /* NSP_AMD_HPP */

@@ -61,7 +61,33 @@ def build(force: bool = False, jobs: int = 4) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libnsp_amd.so failed")
     os.replace(tmp, LIB)
+    _build_cpp()
     return LIB
+
+
+CPP_LIB = os.path.join(HERE, "libnsp_amd_cpp.so")
+DRIVER = os.path.join(HERE, "nsp_amd_run")
+
+
+def _build_cpp() -> None:
+    """The C++ host API (include/nsp_amd.hpp) as libnsp_amd_cpp.so over
+    libnsp_amd.so, and the nsp_amd_run driver; host code, built with hipcc
+    for the HIP runtime headers and libraries."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    common = ["-O2", "-std=c++17", "-fPIC", "-I", inc, "-Wno-unused-result"]
+    link = ["-L", HERE, "-Wl,-rpath,$ORIGIN"]
+    steps = [
+        [HIPCC, *common, "-shared", "-o", CPP_LIB + ".tmp", os.path.join(CSRC, "..", "cpp", "api.cpp"),
+         *link, "-lnsp_amd"],
+        [HIPCC, *common, "-o", DRIVER + ".tmp", os.path.join(CSRC, "..", "cpp", "driver.cpp"),
+         *link, "-lnsp_amd_cpp", "-lnsp_amd"],
+    ]
+    for cmd, out in zip(steps, (CPP_LIB, DRIVER)):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"build of {os.path.basename(out)} failed")
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

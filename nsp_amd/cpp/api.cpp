@@ -1,0 +1,392 @@
+// C++ host API (include/nsp_amd.hpp) over the C ABI of libnsp_amd.so.
+// Host code only: device memory through the HIP runtime, compute through the
+// library's entry points.
+#include "nsp_amd.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace nspamd {
+
+namespace {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw Error(NSP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// One library handle per host thread and device, on the null stream (the
+// reference's calls are host-synchronous on the legacy stream).
+struct ThreadHandles {
+  std::map<int, nsp_handle_t> by_device;
+  ~ThreadHandles() {
+    for (auto &kv : by_device) nsp_destroy(kv.second);
+  }
+};
+
+nsp_handle_t handle() {
+  thread_local ThreadHandles th;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  auto it = th.by_device.find(dev);
+  if (it != th.by_device.end()) return it->second;
+  nsp_handle_t h = nullptr;
+  const int s = nsp_create(&h);
+  if (s != NSP_OK) throw Error(s, "nsp_create failed");
+  th.by_device[dev] = h;
+  return h;
+}
+
+void check(int status, nsp_handle_t h, const char *what) {
+  if (status != NSP_OK)
+    throw Error(status, std::string(what) + ": " + nsp_status_string(status) + " (" +
+                            (h ? nsp_last_error(h) : "") + ")");
+}
+
+template <typename T>
+T *dalloc(size_t n) {
+  void *p = nullptr;
+  hip_check(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+  return static_cast<T *>(p);
+}
+
+nsp_csr_t view(const CSR *A, bool compacted) {
+  nsp_csr_t v;
+  v.rows = A->n;
+  v.cols = compacted ? A->shrinked_m : A->m;
+  v.nnz = A->nnz;
+  v.row_ptr = A->row;
+  v.col = compacted ? A->shrinked_col : A->col;
+  v.val = A->val;
+  return v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void nsp_spgemm_kernel_hash(sfCSR *a, sfCSR *b, sfCSR *c) {
+  if (!a || !b || !c) throw Error(NSP_ERR_INVALID, "nsp_spgemm_kernel_hash: NULL matrix");
+  nsp_handle_t h = handle();
+  // the reference reads only M, N and the device arrays of a and b; nnz
+  // comes from the row pointer (a->nnz is not trusted, nsp.cu:2448-2488)
+  int a_nnz = 0, b_nnz = 0;
+  hip_check(hipMemcpy(&a_nnz, a->d_rpt + a->M, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+  hip_check(hipMemcpy(&b_nnz, b->d_rpt + b->M, sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+  nsp_csr_t A{a->M, b->M, a_nnz, a->d_rpt, a->d_col, a->d_val};
+  nsp_csr_t B{b->M, b->N, b_nnz, b->d_rpt, b->d_col, b->d_val};
+  nsp_csr_t C{};
+  check(nsp_spgemm(h, NSP_IDX32, NSP_F64, &A, &B, &C, nullptr, nullptr, nullptr), h, "nsp_spgemm_kernel_hash");
+  c->M = a->M;
+  c->N = b->N;
+  c->nnz = (int)C.nnz;
+  c->d_rpt = (int *)C.row_ptr;
+  c->d_col = (int *)C.col;
+  c->d_val = (double *)C.val;
+}
+
+void sfCSR_free_device(sfCSR *c) {
+  if (!c) return;
+  (void)hipFree(c->d_rpt);
+  (void)hipFree(c->d_col);
+  (void)hipFree(c->d_val);
+  c->d_rpt = c->d_col = nullptr;
+  c->d_val = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+namespace CSRm {
+
+CSR *init(int n, int64_t m, int nnz, bool allocate, bool on_the_device, int64_t full_n, int64_t row_shift) {
+  CSR *A = new CSR();
+  A->n = n;
+  A->m = m;
+  A->nnz = nnz;
+  A->full_n = full_n < 0 ? n : full_n;
+  A->full_m = m;
+  A->row_shift = row_shift;
+  A->on_the_device = on_the_device;
+  if (allocate) {
+    if (on_the_device) {
+      A->row = dalloc<int>((size_t)n + 1);
+      A->col = dalloc<int>(nnz);
+      A->val = dalloc<double>(nnz);
+    } else {
+      A->row = new int[(size_t)n + 1];
+      A->col = new int[std::max(nnz, 1)];
+      A->val = new double[std::max(nnz, 1)];
+    }
+  }
+  return A;
+}
+
+void free(CSR *A) {
+  if (!A) return;
+  if (A->on_the_device) {
+    (void)hipFree(A->row);
+    (void)hipFree(A->col);
+    (void)hipFree(A->val);
+  } else {
+    delete[] A->row;
+    delete[] A->col;
+    delete[] A->val;
+  }
+  (void)hipFree(A->shrinked_col);
+  (void)hipFree(A->shrinking_permut);
+  delete A;
+}
+
+void freeStruct(CSR *A) { delete A; }
+
+CSR *copyToDevice(const CSR *A) {
+  CSR *D = init(A->n, A->m, A->nnz, true, true, A->full_n, A->row_shift);
+  D->full_m = A->full_m;
+  hip_check(hipMemcpy(D->row, A->row, ((size_t)A->n + 1) * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+  hip_check(hipMemcpy(D->col, A->col, (size_t)A->nnz * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+  hip_check(hipMemcpy(D->val, A->val, (size_t)A->nnz * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+  return D;
+}
+
+CSR *copyToHost(const CSR *A) {
+  CSR *H = init(A->n, A->m, A->nnz, true, false, A->full_n, A->row_shift);
+  H->full_m = A->full_m;
+  hip_check(hipMemcpy(H->row, A->row, ((size_t)A->n + 1) * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+  hip_check(hipMemcpy(H->col, A->col, (size_t)A->nnz * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
+  hip_check(hipMemcpy(H->val, A->val, (size_t)A->nnz * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+  return H;
+}
+
+double norm_fro(const CSR *A) {
+  std::vector<double> v((size_t)A->nnz);
+  if (A->on_the_device)
+    hip_check(hipMemcpy(v.data(), A->val, v.size() * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+  else
+    std::memcpy(v.data(), A->val, v.size() * sizeof(double));
+  double s = 0.0;
+  for (double x : v) s += x * x;
+  return std::sqrt(s);
+}
+
+}  // namespace CSRm
+
+// ---------------------------------------------------------------------------
+bool shrink_col(CSR *A, CSR *P) {
+  if (A->shrinked_flag) return false;
+  if (!A->on_the_device) throw Error(NSP_ERR_INVALID, "shrink_col: A must be on the device");
+  nsp_handle_t h = handle();
+  const int64_t l = (int64_t)(P ? P->n : A->n) - 1;
+  const int64_t lo = A->col_hi > A->col_lo ? A->col_lo : -A->row_shift;
+  const int64_t hi = A->col_hi > A->col_lo ? A->col_hi : A->m - A->row_shift;
+  const int64_t cap = std::max(hi, l + 1) - std::min<int64_t>(lo, 0);
+  int *perm = dalloc<int>((size_t)cap);
+  int *new_col = dalloc<int>((size_t)A->nnz);
+  int64_t m = 0;
+  const int s = nsp_shrink_cols(h, A->nnz, A->col, l, lo, hi, new_col, perm, cap, &m);
+  if (s != NSP_OK) {
+    (void)hipFree(perm);
+    (void)hipFree(new_col);
+    check(s, h, "shrink_col");
+  }
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  A->shrinked_col = new_col;
+  A->shrinked_m = (int)m;
+  A->shrinking_permut = perm;
+  A->shrinking_permut_len = (int)m;
+  A->shrinked_flag = true;
+  return true;
+}
+
+CSR *get_shrinked_matrix(CSR *A, CSR *P) {
+  shrink_col(A, P);
+  CSR *S = CSRm::init(A->n, A->shrinked_m, A->nnz, false, true, A->full_n, A->row_shift);
+  S->row = A->row;
+  S->col = A->shrinked_col;
+  S->val = A->val;
+  return S;
+}
+
+// ---------------------------------------------------------------------------
+CSR *nsparseMGPU_version0(CSR *A, CSR *P, csrlocinfo * /*Plocal*/, double &loc_time) {
+  if (!A || !P || !A->on_the_device || !P->on_the_device)
+    throw Error(NSP_ERR_INVALID, "nsparseMGPU_version0: A and P must be on the device");
+  if (A->m != P->n) throw Error(NSP_ERR_INVALID, "nsparseMGPU_version0: A->m != P->n");
+  nsp_handle_t h = handle();
+  nsp_csr_t a = view(A, false), b = view(P, false), c{};
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  const auto t0 = std::chrono::steady_clock::now();
+  check(nsp_spgemm(h, NSP_IDX32, NSP_F64, &a, &b, &c, nullptr, nullptr, nullptr), h, "nsparseMGPU_version0");
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  loc_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  CSR *C = CSRm::init(A->n, P->m, (int)c.nnz, false, true, A->full_n, A->row_shift);
+  C->row = (int *)c.row_ptr;
+  C->col = (int *)c.col;
+  C->val = (double *)c.val;
+  return C;
+}
+
+CSR *nsparseMGPU_version0(CSR *A, CSR *P) {
+  double t = 0.0;
+  return nsparseMGPU_version0(A, P, nullptr, t);
+}
+
+CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local_n, CSR **A_out, CSR **P_out) {
+  if (!Alocal || !Pfull || !Alocal->on_the_device || !Pfull->on_the_device)
+    throw Error(NSP_ERR_INVALID, "nsparseMGPU_version1: matrices must be on the device");
+  nsp_handle_t h = handle();
+  const int64_t r0 = Alocal->row_shift;
+  CSR Ploc;
+  Ploc.n = p_local_n < 0 ? Alocal->n : p_local_n;
+  // A's compaction: completedP rows = shrinking_permut(A) + r0
+  if (Alocal->col_hi <= Alocal->col_lo) {
+    Alocal->col_lo = -r0;
+    Alocal->col_hi = Pfull->n - r0;
+  }
+  CSR *A_ = get_shrinked_matrix(Alocal, &Ploc);
+  const int nrows = Alocal->shrinking_permut_len;
+  // completedP: gather of P rows perm + r0, columns re-shifted to the rank frame
+  nsp_csr_t pf = view(Pfull, false);
+  CSR *cP = CSRm::init(nrows, Pfull->m, 0, false, true, Pfull->n, r0);
+  cP->row = dalloc<int>((size_t)nrows + 1);
+  int64_t nnz = 0;
+  check(nsp_gather_rows_count(h, NSP_IDX32, &pf, nrows, Alocal->shrinking_permut, r0, cP->row, &nnz), h,
+        "completedP rows");
+  cP->nnz = (int)nnz;
+  cP->col = dalloc<int>((size_t)nnz);
+  cP->val = dalloc<double>((size_t)nnz);
+  nsp_csr_t out = view(cP, false);
+  check(nsp_gather_rows_fill(h, NSP_IDX32, NSP_F64, &pf, nrows, Alocal->shrinking_permut, r0, r0, &out), h,
+        "completedP fill");
+  cP->col_lo = -r0;
+  cP->col_hi = Pfull->m - r0;
+  CSR *P_ = get_shrinked_matrix(cP, nullptr);
+  CSR *C = nsparseMGPU_version0(A_, P_, nullptr, loc_time);
+  if (A_out) {
+    *A_out = A_;
+  } else {
+    CSRm::freeStruct(A_);  // a view of Alocal's arrays
+  }
+  if (P_out) {
+    // P_ owns completedP's arrays from here on
+    P_->row = cP->row; P_->val = cP->val; P_->col = cP->shrinked_col;
+    P_->shrinking_permut = cP->shrinking_permut;
+    P_->shrinking_permut_len = cP->shrinking_permut_len;
+    cP->row = nullptr; cP->val = nullptr; cP->shrinked_col = nullptr; cP->shrinking_permut = nullptr;
+    *P_out = P_;
+  } else {
+    CSRm::freeStruct(P_);  // a view of completedP's arrays
+  }
+  CSRm::free(cP);
+  return C;
+}
+
+// ---------------------------------------------------------------------------
+namespace {
+
+struct CooRec {
+  int64_t i, j;
+  double v;
+};
+
+// first record whose row is >= row1 (1-based rows, file sorted by row)
+int64_t lower_bound_row(FILE *f, int64_t hdr, int64_t nt, int64_t row1) {
+  int64_t lo = 0, hi = nt;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    CooRec r;
+    if (fseeko(f, (off_t)(hdr + mid * (int64_t)sizeof(CooRec)), SEEK_SET) != 0 || fread(&r, sizeof r, 1, f) != 1)
+      return -1;
+    if (r.i < row1) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+}  // namespace
+
+CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type, int64_t *nr_out, int64_t *nc_out,
+                        int64_t *nt_out) {
+  if (nprocs < 1 || rank < 0 || rank >= nprocs) throw Error(NSP_ERR_INVALID, "read_binary_matrix: bad rank");
+  FILE *f = fopen(path, "rb");
+  if (!f) throw Error(NSP_ERR_INVALID, std::string("read_binary_matrix: cannot open ") + path);
+  int64_t h[3] = {0, 0, 0};
+  const int nh = header_type ? 3 : 2;
+  if (fread(h, 8, nh, f) != (size_t)nh) {
+    fclose(f);
+    throw Error(NSP_ERR_INVALID, "read_binary_matrix: short header");
+  }
+  const int64_t nr = h[0], nc = header_type ? h[1] : h[0], nt = header_type ? h[2] : h[1];
+  const int64_t hdr = 8 * nh;
+  // the rank's rows (READ_MATRIX_MPI.cpp:256-270): nr/nprocs each, the rest to the last
+  const int64_t per = nr / nprocs, first = per * rank, last = rank == nprocs - 1 ? nr : per * (rank + 1);
+  const int64_t b = lower_bound_row(f, hdr, nt, first + 1), e = lower_bound_row(f, hdr, nt, last + 1);
+  if (b < 0 || e < 0) {
+    fclose(f);
+    throw Error(NSP_ERR_INVALID, "read_binary_matrix: truncated file");
+  }
+  std::vector<CooRec> rec((size_t)(e - b));
+  if (!rec.empty() && (fseeko(f, (off_t)(hdr + b * (int64_t)sizeof(CooRec)), SEEK_SET) != 0 ||
+                       fread(rec.data(), sizeof(CooRec), rec.size(), f) != rec.size())) {
+    fclose(f);
+    throw Error(NSP_ERR_INVALID, "read_binary_matrix: truncated records");
+  }
+  fclose(f);
+  // the block spans the rows it holds (distribute.cu:1096): first to last row present
+  const int64_t r_first = rec.empty() ? first : rec.front().i - 1;
+  const int64_t r_last = rec.empty() ? first - 1 : rec.back().i - 1;
+  const int n = (int)(r_last - r_first + 1), nnz = (int)rec.size();
+  CSR *A = CSRm::init(n, nc, nnz, true, false, nr, r_first);
+  std::fill(A->row, A->row + n + 1, 0);
+  for (const CooRec &r : rec) A->row[r.i - 1 - r_first + 1]++;
+  for (int i = 0; i < n; i++) A->row[i + 1] += A->row[i];
+  for (int q = 0; q < nnz; q++) {
+    A->col[q] = (int)(rec[q].j - 1 - r_first);  // rank frame (distribute.cu:1097-1105)
+    A->val[q] = rec[q].v;
+  }
+  // rows in column order (check_and_fix_order): a stable sort per row
+  std::vector<std::pair<int, double>> tmp;
+  for (int i = 0; i < n; i++) {
+    const int p0 = A->row[i], p1 = A->row[i + 1];
+    bool sorted = true;
+    for (int p = p0 + 1; p < p1 && sorted; p++) sorted = A->col[p - 1] <= A->col[p];
+    if (sorted) continue;
+    tmp.clear();
+    for (int p = p0; p < p1; p++) tmp.emplace_back(A->col[p], A->val[p]);
+    std::stable_sort(tmp.begin(), tmp.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    for (int p = p0; p < p1; p++) {
+      A->col[p] = tmp[p - p0].first;
+      A->val[p] = tmp[p - p0].second;
+    }
+  }
+  if (nr_out) *nr_out = nr;
+  if (nc_out) *nc_out = nc;
+  if (nt_out) *nt_out = nt;
+  return A;
+}
+
+void write_binary_matrix(const char *path, const CSR *A, int64_t col_shift) {
+  if (A->on_the_device) throw Error(NSP_ERR_INVALID, "write_binary_matrix: host matrix expected");
+  FILE *f = fopen(path, "wb");
+  if (!f) throw Error(NSP_ERR_INVALID, std::string("write_binary_matrix: cannot open ") + path);
+  const int64_t h[3] = {A->n, A->m, A->nnz};
+  bool ok = fwrite(h, 8, 3, f) == 3;
+  std::vector<CooRec> buf;
+  buf.reserve(1 << 16);
+  for (int i = 0; i < A->n && ok; i++) {
+    for (int p = A->row[i]; p < A->row[i + 1]; p++) {
+      buf.push_back(CooRec{(int64_t)i + 1, (int64_t)A->col[p] + col_shift + 1, A->val[p]});
+      if (buf.size() == buf.capacity()) {
+        ok = fwrite(buf.data(), sizeof(CooRec), buf.size(), f) == buf.size();
+        buf.clear();
+      }
+    }
+  }
+  if (ok && !buf.empty()) ok = fwrite(buf.data(), sizeof(CooRec), buf.size(), f) == buf.size();
+  fclose(f);
+  if (!ok) throw Error(NSP_ERR_INVALID, std::string("write_binary_matrix: write failed on ") + path);
+}
+
+}  // namespace nspamd

@@ -1,0 +1,137 @@
+// nsp_amd_run: command-line driver of the C++ API, following the flow of the
+// reference's main.cu (read -> split -> local product -> report nnz and
+// ||C||_F, main.cu:223-241) on one device.
+//
+//   nsp_amd_run product A.bin B.bin [C.bin]
+//       C = A*B through nsp_spgemm_kernel_hash (sfCSR interface)
+//   nsp_amd_run local A.bin P.bin nprocs rank [C.bin]
+//       rank `rank` of `nprocs`: its row block of A times the P rows it needs
+//       (nsparseMGPU_version1 without MPI: P is read whole onto the device)
+//   nsp_amd_run split A.bin nprocs rank out.bin
+//       the reader/splitter alone (host only): the rank's block, columns in
+//       the rank frame, written back as binary COO
+//
+// Matrices are binary COO files (header {nr, nc, nt}, then {i, j, v}
+// records, 1-based, sorted by row).  Prints one JSON line; C.bin, when given,
+// receives C in the same format (columns as the product returns them).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nsp_amd.hpp"
+
+using namespace nspamd;
+
+static void write_device_csr(const char *path, CSR *C) {
+  CSR *H = CSRm::copyToHost(C);
+  write_binary_matrix(path, H, 0);
+  CSRm::free(H);
+}
+
+static int run_product(int argc, char **argv) {
+  if (argc < 4) return 2;
+  CSR *A = read_binary_matrix(argv[2], 1, 0);
+  CSR *B = read_binary_matrix(argv[3], 1, 0);
+  CSR *Ad = CSRm::copyToDevice(A), *Bd = CSRm::copyToDevice(B);
+  sfCSR a{}, b{}, c{};
+  a.M = (unsigned)Ad->n;
+  a.N = (unsigned)Ad->m;
+  a.nnz = Ad->nnz;
+  a.d_rpt = Ad->row;
+  a.d_col = Ad->col;
+  a.d_val = Ad->val;
+  b.M = (unsigned)Bd->n;
+  b.N = (unsigned)Bd->m;
+  b.nnz = Bd->nnz;
+  b.d_rpt = Bd->row;
+  b.d_col = Bd->col;
+  b.d_val = Bd->val;
+  const auto t0 = std::chrono::steady_clock::now();
+  nsp_spgemm_kernel_hash(&a, &b, &c);
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  CSR Cv;
+  Cv.n = (int)c.M;
+  Cv.m = c.N;
+  Cv.nnz = c.nnz;
+  Cv.row = c.d_rpt;
+  Cv.col = c.d_col;
+  Cv.val = c.d_val;
+  Cv.on_the_device = true;
+  const double l2 = CSRm::norm_fro(&Cv);
+  if (argc > 4) write_device_csr(argv[4], &Cv);
+  printf("{\"mode\": \"product\", \"rows\": %u, \"cols\": %u, \"nnz\": %d, \"l2norm\": %.17g, \"time_s\": %.6f}\n",
+         c.M, c.N, c.nnz, l2, t);
+  sfCSR_free_device(&c);
+  CSRm::free(Ad);
+  CSRm::free(Bd);
+  CSRm::free(A);
+  CSRm::free(B);
+  return 0;
+}
+
+static int run_local(int argc, char **argv) {
+  if (argc < 6) return 2;
+  const int nprocs = atoi(argv[4]), rank = atoi(argv[5]);
+  int64_t nr = 0;
+  CSR *A = read_binary_matrix(argv[2], nprocs, rank, 1, &nr);
+  CSR *Pblk = read_binary_matrix(argv[3], nprocs, rank);
+  CSR *P = read_binary_matrix(argv[3], 1, 0);
+  CSR *Ad = CSRm::copyToDevice(A), *Pd = CSRm::copyToDevice(P);
+  double loc_time = 0.0;
+  CSR *A_ = nullptr, *P_ = nullptr;
+  CSR *C = nsparseMGPU_version1(Ad, Pd, loc_time, Pblk->n, &A_, &P_);
+  const double l2 = CSRm::norm_fro(C);
+  if (argc > 6) write_device_csr(argv[6], C);
+  printf("{\"mode\": \"local\", \"nprocs\": %d, \"rank\": %d, \"row_shift\": %lld, \"rows\": %d, "
+         "\"shrinked_m_A\": %d, \"completedP_rows\": %d, \"shrinked_m_P\": %d, \"nnz\": %d, "
+         "\"l2norm\": %.17g, \"loc_time\": %.6f}\n",
+         nprocs, rank, (long long)A->row_shift, C->n, A_->shrinked_m ? A_->shrinked_m : (int)A_->m, P_->n,
+         (int)P_->m, C->nnz, l2, loc_time);
+  CSRm::free(C);
+  CSRm::free(P_);
+  CSRm::freeStruct(A_);
+  CSRm::free(Ad);
+  CSRm::free(Pd);
+  CSRm::free(A);
+  CSRm::free(P);
+  CSRm::free(Pblk);
+  return 0;
+}
+
+static int run_split(int argc, char **argv) {
+  if (argc < 6) return 2;
+  const int nprocs = atoi(argv[3]), rank = atoi(argv[4]);
+  int64_t nr = 0, nc = 0, nt = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  CSR *A = read_binary_matrix(argv[2], nprocs, rank, 1, &nr, &nc, &nt);
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  write_binary_matrix(argv[5], A, 0);
+  printf("{\"mode\": \"split\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %d, \"nnz\": %d, "
+         "\"row_shift\": %lld, \"read_s\": %.6f}\n",
+         (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, (long long)A->row_shift, t);
+  CSRm::free(A);
+  return 0;
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: %s product A.bin B.bin [C.bin] | local A.bin P.bin nprocs rank [C.bin]\n", argv[0]);
+    return 2;
+  }
+  try {
+    int rc = 2;
+    if (!strcmp(argv[1], "product")) rc = run_product(argc, argv);
+    else if (!strcmp(argv[1], "local")) rc = run_local(argc, argv);
+    else if (!strcmp(argv[1], "split")) rc = run_split(argc, argv);
+    if (rc == 2) fprintf(stderr, "bad arguments\n");
+    return rc;
+  } catch (const Error &e) {
+    fprintf(stderr, "nsp_amd_run: %s (status %d)\n", e.what(), e.status);
+    return 1;
+  }
+}

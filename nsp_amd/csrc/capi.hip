@@ -175,6 +175,9 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
     h->num_table_factor = (int)value;
   } else if (!std::strcmp(name, "timing")) {
     h->timing = value != 0;
+  } else if (!std::strcmp(name, "heavy_blocks")) {
+    if (value < 0 || value > 65536) return nsp::set_error(h, NSP_ERR_INVALID, "heavy_blocks must be in [0, 65536]");
+    h->heavy_blocks = (int)value;
   } else if (!std::strcmp(name, "grid_cap")) {
     if (value < 1 || value > 64) return nsp::set_error(h, NSP_ERR_INVALID, "grid_cap must be in [1, 64]");
     h->grid_cap = (int)value;

@@ -20,6 +20,9 @@ template <> struct Log2<1> { static constexpr int value = 0; };
 __device__ __forceinline__ int32_t lds_ld(const int32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+__device__ __forceinline__ uint32_t lds_ld(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // Orders LDS accesses of one wavefront.  A wave's LDS instructions execute in
 // issue order, so a compiler barrier is all that is needed between a write by

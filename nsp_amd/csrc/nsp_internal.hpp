@@ -65,6 +65,7 @@ struct Handle {
   int sym_rows_per_wave = 2;
   int num_small_rows_per_wave = 4;
   int num_rows_per_wave = 2;
+  int heavy_blocks = 0;  // 0: one block per CU
   int num_table_factor = 2;
   bool timing = false;
   bool timing_pending = false;

@@ -555,12 +555,210 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
 }
 
 // ---------------------------------------------------------------------------
+// Heavy rows (more than 16384 intermediate products, or more than 4096
+// entries in C): one block of 1024 threads per row, a bitmap of the row's
+// columns in LDS instead of a hash table.  The row's column range comes from
+// the first and last columns of the B rows it touches (B rows are sorted);
+// ranges wider than one bitmap window are covered window by window, each
+// window a new walk over the row's products.  (The reference's heavy-row
+// fallback is a global-memory hash per row, nsparse kernel_spgemm_hash_d.cu
+// 604-678 and 1037-1165, and 8192-column window passes in nsp.cu.)
+constexpr int kHeavyBS = 1024;
+constexpr int kHeavyNW = kHeavyBS / 64;
+constexpr int64_t kSymHeavyWin = 1 << 20;   // columns per symbolic window (128 KB of bits)
+constexpr int64_t kNumHeavyWin = 1 << 19;   // columns per numeric window (64 KB bits + 64 KB ranks)
+
+// Smallest and largest column of a row's products.
+template <int BS>
+__device__ __forceinline__ void heavy_col_range(int64_t a0, int64_t a1, const int64_t *__restrict__ ab_b0,
+                                                const int32_t *__restrict__ ab_len,
+                                                const int32_t *__restrict__ bcol, int64_t *s_red,
+                                                int64_t &lo, int64_t &hi) {
+  int64_t mn = INT64_MAX, mx = -1;
+  for (int64_t p = a0 + threadIdx.x; p < a1; p += BS) {
+    const int len = ab_len[p];
+    if (len > 0) {
+      const int64_t b0 = ab_b0[p];
+      const int64_t f = bcol[b0], l = bcol[b0 + len - 1];
+      mn = f < mn ? f : mn;
+      mx = l > mx ? l : mx;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int64_t a = __shfl_xor(mn, o), b = __shfl_xor(mx, o);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_red[2 * w] = mn;
+    s_red[2 * w + 1] = mx;
+  }
+  __syncthreads();
+  lo = INT64_MAX;
+  hi = -1;
+  for (int q = 0; q < BS / 64; q++) {
+    lo = s_red[2 * q] < lo ? s_red[2 * q] : lo;
+    hi = s_red[2 * q + 1] > hi ? s_red[2 * q + 1] : hi;
+  }
+  __syncthreads();
+}
+
+template <int R>
+__global__ __launch_bounds__(kHeavyBS) void k_sym_heavy(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                        const Info *__restrict__ info,
+                                                        const int32_t *__restrict__ perm,
+                                                        const int64_t *__restrict__ arpt,
+                                                        const int64_t *__restrict__ ab_b0,
+                                                        const int32_t *__restrict__ ab_len,
+                                                        const int32_t *__restrict__ bcol,
+                                                        int32_t *__restrict__ row_nz) {
+  constexpr int NWORD = (int)(kSymHeavyWin / 32);
+  __shared__ uint32_t bm[NWORD];
+  __shared__ EntryScratch<64, float> ent[kHeavyNW];
+  __shared__ int64_t s_red[2 * kHeavyNW];
+  __shared__ int s_cnt[kHeavyNW + 1];
+  int64_t count, off;
+  bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const int32_t r = perm ? perm[off + t] : (int32_t)t;
+    const int64_t a0 = arpt[r], a1 = arpt[r + 1];
+    int64_t lo, hi;
+    heavy_col_range<kHeavyBS>(a0, a1, ab_b0, ab_len, bcol, s_red, lo, hi);
+    int64_t total = 0;
+    for (int64_t w0 = lo; w0 <= hi; w0 += kSymHeavyWin) {
+      for (int q = threadIdx.x; q < NWORD; q += kHeavyBS) bm[q] = 0;
+      __syncthreads();
+      row_products<64, R, false, float>(
+          lane, a0, a1, 64 * w, 64 * kHeavyNW, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[w],
+          [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              const int64_t c = (int64_t)key[u] - w0;
+              if (ok[u] && c >= 0 && c < kSymHeavyWin) atomicOr(&bm[c >> 5], 1u << (c & 31));
+            }
+          });
+      __syncthreads();
+      int c = 0;
+      for (int q = threadIdx.x; q < NWORD; q += kHeavyBS) c += __popc(bm[q]);
+      c = group_sum<64>(c);
+      if (lane == 0) s_cnt[w] = c;
+      __syncthreads();
+      for (int q = 0; q < kHeavyNW; q++) total += s_cnt[q];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) row_nz[r] = (int32_t)total;
+  }
+}
+
+template <typename V, int R>
+__global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                        const Info *__restrict__ info,
+                                                        const int32_t *__restrict__ perm,
+                                                        const int64_t *__restrict__ arpt,
+                                                        const int64_t *__restrict__ ab_b0,
+                                                        const V *__restrict__ aval,
+                                                        const int32_t *__restrict__ ab_len,
+                                                        const int32_t *__restrict__ bcol,
+                                                        const V *__restrict__ bval,
+                                                        const int64_t *__restrict__ crpt,
+                                                        int32_t *__restrict__ ccol,
+                                                        V *__restrict__ cval) {
+  constexpr int NWORD = (int)(kNumHeavyWin / 32);
+  constexpr int WPT = NWORD / kHeavyBS;  // consecutive words per thread in the rank scan
+  __shared__ uint32_t bm[NWORD];
+  __shared__ uint32_t wbase[NWORD];  // row position of the first column of each word
+  __shared__ EntryScratch<64, V> ent[kHeavyNW];
+  __shared__ int64_t s_red[2 * kHeavyNW];
+  __shared__ int s_scan[kHeavyNW + 1];
+  int64_t count, off;
+  bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int64_t t = blockIdx.x; t < count; t += gridDim.x) {
+    const int32_t r = perm ? perm[off + t] : (int32_t)t;
+    if (crpt[r + 1] > info->cap) {  // C buffers too small: skip the row, flag it
+      if (threadIdx.x == 0)
+        atomicOr((unsigned int *)&const_cast<Info *>(info)->error, (unsigned int)NSP_ERR_OVERFLOW);
+      continue;
+    }
+    const int64_t a0 = arpt[r], a1 = arpt[r + 1];
+    const int64_t c0 = crpt[r], nzr = crpt[r + 1] - c0;
+    int64_t lo, hi;
+    heavy_col_range<kHeavyBS>(a0, a1, ab_b0, ab_len, bcol, s_red, lo, hi);
+    int64_t base = 0;  // entries of the row placed by earlier windows
+    for (int64_t w0 = lo; w0 <= hi; w0 += kNumHeavyWin) {
+      for (int q = threadIdx.x; q < NWORD; q += kHeavyBS) bm[q] = 0;
+      __syncthreads();
+      row_products<64, R, false, V>(
+          lane, a0, a1, 64 * w, 64 * kHeavyNW, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[w],
+          [&](const int32_t(&key)[R], const V(&)[R], const bool(&ok)[R], auto &&...) {
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              const int64_t c = (int64_t)key[u] - w0;
+              if (ok[u] && c >= 0 && c < kNumHeavyWin) atomicOr(&bm[c >> 5], 1u << (c & 31));
+            }
+          });
+      __syncthreads();
+      // ranks: exclusive scan of per-word popcounts (WPT consecutive words per thread)
+      const int q0 = threadIdx.x * WPT;
+      int mine = 0;
+#pragma unroll
+      for (int q = 0; q < WPT; q++) mine += __popc(bm[q0 + q]);
+      int wtotal;
+      int run = block_excl_scan<kHeavyBS, int>(mine, s_scan, &wtotal);
+      // sorted columns of the window, values zeroed for the accumulation
+#pragma unroll 1
+      for (int q = 0; q < WPT; q++) {
+        uint32_t x = bm[q0 + q];
+        wbase[q0 + q] = (uint32_t)run;
+        while (x) {
+          const int b = __ffs(x) - 1;
+          x &= x - 1;
+          if (base + run < nzr) {  // the symbolic count bounds the row
+            ccol[c0 + base + run] = (int32_t)(w0 + (int64_t)(q0 + q) * 32 + b);
+            cval[c0 + base + run] = (V)0;
+          }
+          run++;
+        }
+      }
+      __threadfence();
+      __syncthreads();
+      row_products<64, R, true, V>(
+          lane, a0, a1, 64 * w, 64 * kHeavyNW, ab_b0, ab_len, aval, bcol, bval, &ent[w],
+          [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              const int64_t c = (int64_t)key[u] - w0;
+              if (ok[u] && c >= 0 && c < kNumHeavyWin) {
+                const int q = (int)(c >> 5), b = (int)(c & 31);
+                const int64_t rk = base + wbase[q] + __popc(bm[q] & ((1u << b) - 1u));
+                if (rk < nzr) unsafeAtomicAdd(&cval[c0 + rk], prod[u]);
+              }
+            }
+          });
+      base += wtotal;
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launch tables
 static int grid_for(const Handle *h, int64_t items, int per_block, int blocks_per_cu) {
   int64_t need = (items + per_block - 1) / per_block;
   int64_t cap = (int64_t)h->num_cu * blocks_per_cu;
   if (need < 1) need = 1;
   return (int)std::min(need, cap);
+}
+
+// Heavy-row kernels: one 1024-thread block per row, at most "heavy_blocks"
+// blocks in flight (default one per CU; fewer keeps the C rows being
+// accumulated inside L2).
+static int heavy_grid(const Handle *h, int64_t rows) {
+  const int64_t cap = h->heavy_blocks > 0 ? h->heavy_blocks : h->num_cu;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(rows, cap));
 }
 
 constexpr int kSymR = 8;  // B loads in flight per lane, symbolic
@@ -597,6 +795,10 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
     case 5: SYM_RPW(1024); break;
     case 6: SYM_BLOCK(256, 2048); break;
     case 7: SYM_BLOCK(512, 16384); break;
+    case 8:
+      hipLaunchKernelGGL((k_sym_heavy<kSymR>), dim3(heavy_grid(h, count)), dim3(kHeavyBS), 0, st, lo, bin,
+                         direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz);
+      break;
     default: break;
   }
 #undef SYM_RPW
@@ -641,6 +843,10 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
     case 5: NUM_GROUP(64, 512, 2); break;
     case 6: NUM_BLOCK(256, 2048, 2); break;
     case 7: NUM_BLOCK(512, 4096, 2); break;
+    case 8:
+      hipLaunchKernelGGL((k_num_heavy<V, kNumR>), dim3(heavy_grid(h, count)), dim3(kHeavyBS), 0, st, lo, bin,
+                         direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval, crpt, ccol, cval);
+      break;
     default: break;
   }
 #undef NUM_F
@@ -798,9 +1004,6 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
   NSP_TRY(run_rowinfo<RowT>(h, A, B, bt));
   NSP_TRY(read_info(h));
   const Info &hi = *h->h_info;
-  if (hi.count_sym[kHeavyBin] > 0)
-    return set_error(h, NSP_ERR_UNSUPPORTED,
-                     "rows with more than 16384 intermediate products need the column-window path");
   {
     Launch L[kBins];
     const int n = plan_launches(hi.count_sym, MA, L);
@@ -839,8 +1042,6 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
     h->stats.num_bin[b] = hi.count_num[b];
     h->num_count[b] = hi.count_num[b];
   }
-  if (hi.count_num[kHeavyBin] > 0)
-    return set_error(h, NSP_ERR_UNSUPPORTED, "rows of C with more than 4096 entries need the column-window path");
   *c_nnz = h->stats.nnz;
   h->have_plan = true;
   h->plan_rows = MA;
@@ -944,12 +1145,12 @@ static int graph_record(Handle *g, const nsp_csr_t *A, const nsp_csr_t *B, const
   const int64_t MA = A->rows;
   NSP_TRY(run_rowinfo<RowT>(g, A, B, bt, C->nnz));
   launch_scatter(g, MA, g->d_info->count_sym, g->d_info->cursor_sym, g->perm_sym);
-  for (int b = 0; b < kHeavyBin; b++)
+  for (int b = 0; b < kBins; b++)
     launch_sym_bin(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, g->ab_len, B->col);
   hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)(MA / kScanTile + 1)), dim3(kScanBS), 0, g->stream, MA, g->row_nz,
                      (RowT *)C->row_ptr, g->c_rpt64, g->cls, g->tile_status, g->d_info);
   launch_scatter(g, MA, g->d_info->count_num, g->d_info->cursor_num, g->perm_num);
-  for (int b = 0; b < kHeavyBin; b++)
+  for (int b = 0; b < kBins; b++)
     launch_num_bin<V>(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len,
                       B->col, (const V *)B->val, C->col, (V *)C->val);
   NSP_HIP(g, hipGetLastError());
@@ -974,8 +1175,11 @@ int graph_create(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, 
   g->device = h->device;
   g->num_cu = h->num_cu;
   g->grid_cap = h->grid_cap;
+  g->sym_small_rows_per_wave = h->sym_small_rows_per_wave;
   g->sym_rows_per_wave = h->sym_rows_per_wave;
+  g->num_small_rows_per_wave = h->num_small_rows_per_wave;
   g->num_rows_per_wave = h->num_rows_per_wave;
+  g->heavy_blocks = h->heavy_blocks;
   g->num_table_factor = h->num_table_factor;
   gp->rows = A->rows;
   gp->launch_stream = h->stream;
@@ -1037,8 +1241,6 @@ int graph_result(GraphPlan *gp, int64_t *nnz) {
   NSP_HIP(g, hipStreamSynchronize(gp->launch_stream));
   const Info &hi = *g->h_info;
   if (nnz) *nnz = hi.nnz;
-  if (hi.count_sym[kHeavyBin] > 0 || hi.count_num[kHeavyBin] > 0)
-    return set_error(g, NSP_ERR_UNSUPPORTED, "graph: rows beyond 16384 products / 4096 entries need the plain path");
   if (hi.error) return set_error(g, (int)hi.error, "graph: C exceeds the buffer capacity or the index width");
   return NSP_OK;
 }

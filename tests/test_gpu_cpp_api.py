@@ -1,0 +1,84 @@
+"""The C++ host API (include/nsp_amd.hpp) end to end through the nsp_amd_run
+driver: binary COO files in, the product on the GPU, C written back and
+compared with the oracle (structure bit-exact, values 1e-12 componentwise).
+
+  product: nsp_spgemm_kernel_hash on sfCSR descriptors
+  local:   read -> split -> nsparseMGPU_version1 (completedP gather, both
+           compactions, C = A_ * P_) for every rank of a 4-way split
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from nsp_amd import gen
+from tests.helpers import TOL
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "nsp_amd", "nsp_amd_run")
+
+
+def drive(*args):
+    print("  driver:", " ".join(str(a) for a in args), flush=True)
+    r = subprocess.run([DRIVER, *[str(a) for a in args]], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def read_c(path):
+    hdr = np.fromfile(path, dtype=np.int64, count=3)
+    rec = np.fromfile(path, dtype=[("i", "<i8"), ("j", "<i8"), ("v", "<f8")], offset=24)
+    n = int(hdr[0])
+    rpt = np.zeros(n + 1, np.int64)
+    np.add.at(rpt, rec["i"], 1)
+    return np.cumsum(rpt), (rec["j"] - 1).astype(np.int32), rec["v"], int(hdr[1])
+
+
+def compare(rpt, col, val, ref):
+    r_rpt, r_col, r_val, r_abs = ref
+    np.testing.assert_array_equal(rpt, r_rpt)
+    np.testing.assert_array_equal(col, r_col)
+    den = np.where(r_abs > 0, r_abs, 1.0)
+    err = float(np.max(np.abs(val - r_val) / den)) if val.size else 0.0
+    assert err <= TOL["f64"], err
+    return err
+
+
+def test_kernel_hash_sfcsr(orc, tmp_path):
+    print("case: cpp product", flush=True)
+    A = gen.stencil_27pt(14, seed=3)
+    B = gen.random_csr(A.ncols, 900, 0.01, seed=4)
+    pa, pb, pc = (str(tmp_path / n) for n in ("a.bin", "b.bin", "c.bin"))
+    orc.write_binary_coo(pa, A)
+    orc.write_binary_coo(pb, B)
+    meta = drive("product", pa, pb, pc)
+    ref = orc.spgemm(A, B)
+    rpt, col, val, ncols = read_c(pc)
+    assert meta["nnz"] == int(ref[0][-1]) and ncols == B.ncols
+    compare(rpt, col, val, ref)
+    assert abs(meta["l2norm"] - np.linalg.norm(ref[2])) <= 1e-10 * np.linalg.norm(ref[2])
+
+
+@pytest.mark.parametrize("rank", [0, 1, 2, 3])
+def test_local_share_four_ranks(orc, tmp_path, rank):
+    """27-point stencil 16^3 as A and P, split 4 ways: rank 0's compaction is
+    the identity, middle ranks have halos on both sides."""
+    print("case: cpp local rank", rank, flush=True)
+    m, nprocs = 16, 4
+    A = gen.stencil_27pt(m, seed=6)
+    pa, pc = str(tmp_path / "a.bin"), str(tmp_path / f"c{rank}.bin")
+    orc.write_binary_coo(pa, A)
+    meta = drive("local", pa, pa, nprocs, rank, pc)
+    nr, nc, nt, r0, A_rank = orc.read_binary_coo(pa, nprocs, rank)
+    _, _, _, _, P_rank = orc.read_binary_coo(pa, nprocs, rank)
+    sh = orc.local_share(A_rank, A, r0, P_rank.nrows)
+    ref = orc.spgemm(sh["A_"], sh["P_"])
+    assert meta["row_shift"] == r0
+    assert meta["shrinked_m_A"] == sh["A_"].ncols
+    assert meta["completedP_rows"] == sh["P_"].nrows and meta["shrinked_m_P"] == sh["P_"].ncols
+    assert meta["nnz"] == int(ref[0][-1])
+    rpt, col, val, _ = read_c(pc)
+    compare(rpt, col, val, ref)
