@@ -65,6 +65,21 @@ def algorithmic_bytes(A, B, nnzC, s=8):
     return sym, scan, num
 
 
+TRAFFIC_FILES = {"c2": "spgemm_numeric_stencil128_rank0_traffic.json"}
+
+
+def measured_traffic(workload: str):
+    """HBM bytes per launch of the numeric kernel from the committed counter
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy's), or
+    None when the workload has none."""
+    name = TRAFFIC_FILES.get(workload)
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return int(round(json.load(f)["traffic_bytes"]))
+
+
 def cpu_baseline(A, B, nprod_total, budget_s=20.0):
     """Oracle Gustavson on this host.  Times the full product if a first
     timed slice says it fits the budget, else a leading row block."""
@@ -160,6 +175,8 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
     dX = nsp_amd.DeviceCSR.from_host(X, "f32")
     d_off = torch.from_numpy(off).cuda()
     h = nsp_amd.Handle()
+    from nsp_amd import autotune
+    tuned = autotune.tune(h, dX, dX, d_off, d_off, reps=2, use_cache=False)
     _, nnz = h.batched_symbolic(dX, d_off, dX, d_off)
     nprod = h.stats()["nprod"]
     g = h.graph(dX, dX, capacity=nnz, a_off=d_off, b_off=d_off)
@@ -200,7 +217,7 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
             "rows": X.nrows, "nnz": X.nnz, "nnz_C": nnz, "nprod": nprod,
             "graph_ms": round(t_graph, 3), "eager_batched_ms": round(t_eager, 3),
             "loop_ms": round(t_loop, 3), "speedup_vs_loop": round(t_loop / t_graph, 2),
-            "graph_GFLOPs": round(2.0 * nprod / t_graph / 1e6, 1),
+            "graph_GFLOPs": round(2.0 * nprod / t_graph / 1e6, 1), "autotuned": tuned,
             "structure_equal": bool(same), "max_componentwise_diff": err}
 
 
@@ -247,6 +264,10 @@ def main():
                     help="extra measurements after the headline: fwd_bwd, batched, rmat "
                          "(comma list, or none)")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's pick")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy three times (the reference kernel for "
+                         "scaling HBM byte counters)")
     ap.add_argument("--param", action="append", default=[],
                     help="library tuning parameter name=value (repeatable)")
     args = ap.parse_args()
@@ -260,13 +281,24 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}))
         sys.exit(1)
+    if args.calibrate_copy:
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": x.numel() * 8}), flush=True)
+        return
 
     A, B, desc = WORKLOADS[args.workload]()
     dA = nsp_amd.DeviceCSR.from_host(A, "f64", "i32")
     dB = nsp_amd.DeviceCSR.from_host(B, "f64", "i32")
     h = nsp_amd.Handle()
-    h.set_param("timing", 1)
     params = {}
+    if not args.no_tune:
+        from nsp_amd import autotune
+        params = autotune.tune(h, dA, dB, reps=3, use_cache=False)
+    h.set_param("timing", 1)
     for kv in args.param:
         k, v = kv.split("=")
         h.set_param(k, int(v))
@@ -301,6 +333,7 @@ def main():
     gflops = 2.0 * nprod / (ms / 1e3) / 1e9
 
     sym_b, scan_b, num_b = algorithmic_bytes(A, B, nnz, 8)
+    traffic = measured_traffic(args.workload)
     t_num_avg = float(np.mean(t_num))
     achieved = num_b / (t_num_avg / 1e3) / 1e9
     out = {
@@ -313,7 +346,7 @@ def main():
                    "nprod": nprod, "index": "int32", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "numeric hash-accumulate + in-row sort",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": num_b, "kernel_ms": round(t_num_avg, 4)},
         "phases_ms": {"binning": round(float(np.mean(t_bin)), 4),
                       "symbolic": round(float(np.mean(t_sym)), 4),

@@ -374,3 +374,20 @@ def test_uneven_batch_generator():
             assert seg.size >= 1 and np.all(np.diff(seg) > 0)
     mats = gen.uneven_batch(20, seed=3, nmin=64, nmax=300)
     np.testing.assert_array_equal(mats[7].col, X.row_slice(int(off[7]), int(off[8])).col)
+
+
+def test_autotune_signature_and_cache(tmp_path, monkeypatch):
+    """Host side of the autotuner: bucketed signatures and the JSON cache."""
+    from nsp_amd import autotune
+    assert autotune._bucket([0, 1, 2, 3, 1024]) == [0, 1, 2, 2, 11]
+
+    class M:
+        value_type, index_type = 0, 0
+    s1 = autotune.signature({"sym_bin": [0, 5, 900], "num_bin": [7, 0]}, M, False)
+    s2 = autotune.signature({"sym_bin": [0, 6, 1000], "num_bin": [6, 0]}, M, False)
+    s3 = autotune.signature({"sym_bin": [0, 6, 1000], "num_bin": [6, 0]}, M, True)
+    assert s1 == s2 and s1 != s3
+    monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "x" / "t.json"))
+    autotune._save_cache(autotune.cache_path(), {s1: {"params": {"grid_cap": 4}}})
+    assert autotune._load_cache(autotune.cache_path())[s1]["params"]["grid_cap"] == 4
+    assert autotune._load_cache(str(tmp_path / "missing.json")) == {}
