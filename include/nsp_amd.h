@@ -99,6 +99,12 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *   "num_table_factor"   numeric hash slots per row entry (2, 4)
  *   "heavy_blocks" blocks in flight for rows past 16384 products / 4096
  *                  entries (0 = one per CU)
+ *   "reuse_extents" 1: nsp_spgemm_backward reuses the per-entry B row
+ *                  extents, product offsets and C row-length check of the
+ *                  previous call on this handle when A's column, A's and B's
+ *                  row-pointer and C's buffers are the same (the caller
+ *                  promises the patterns did not change); such calls do not
+ *                  synchronise the host
  *   "timing"       1 records HIP events around each phase (see nsp_stats_t) */
 int nsp_set_param(nsp_handle_t h, const char *name, int64_t value);
 

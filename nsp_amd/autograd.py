@@ -45,6 +45,7 @@ class SpGEMMFixedPattern:
     def __init__(self, A: DeviceCSR, B: DeviceCSR, handle: Handle | None = None):
         self.h = handle or Handle()
         self.hb = Handle(self.h.stream)
+        self.hb.set_param("reuse_extents", 1)  # A's and B's patterns are fixed
         self.A, self.B = A, B
         rpt, nnz = self.h.symbolic(A, B)
         self.C = self.h.numeric(A, B, rpt, nnz)

@@ -5,12 +5,11 @@
 // Every (i, j) a forward product reaches lies in pat(C), so both are sums over
 // the forward's intermediate products; no new structure is created.
 //
-// dA walks A's rows with the forward enumerator (row_products.hpp): C row i
-// goes into an LDS hash (column -> dC_ij), and each product of entry (i,k)
-// adds B_kj * dC_ij into that entry's LDS accumulator.  dB walks B's rows with
-// the cached A^T (rows sorted by i, so the sum order is fixed): for every
-// (i, A_ik) of A^T row k and every j in B_k it binary-searches dC_ij in C
-// row i.  Both are deterministic.
+// dA walks A's rows: C row i goes into an LDS hash (column -> dC_ij) and the
+// lane of entry (i,k) sums B_kj * dC_ij over B row k in order.  dB walks B's
+// entries with the cached A^T (rows sorted by i): for every (i, A_ik) of A^T
+// row k it binary-searches dC_ij in C row i; lanes split A^T's row and a fixed
+// shuffle tree adds their sums.  Both are deterministic.
 //
 // The reference has no backward; its only transpose is cuSPARSE csr2csc
 // (src/CSR.cu:739-797).
@@ -20,7 +19,6 @@
 
 #include "device_utils.hpp"
 #include "nsp_internal.hpp"
-#include "row_products.hpp"
 #include "scan.hpp"
 #include "spgemm_internal.hpp"
 
@@ -189,31 +187,41 @@ __device__ __forceinline__ int table_log(int64_t nz) {
   return log;
 }
 
-// dA: GS lanes per A row; C row i in an LDS table column -> dC.  A launch
-// handles the rows with nz_lo < nz(C_i) <= NS; rows longer than NS set
-// *too_long when too_long is given (the largest tier).
-template <typename V, typename RowT, int GS, int NS, int R>
-__global__ __launch_bounds__(256) void k_grad_a(int64_t MA, int64_t nz_lo, const RowT *__restrict__ arpt,
+// dA: GS lanes per A row, one lane per A entry.  C row i goes into an LDS
+// table column -> dC sized for the row; lane (i,k) walks B row k in order and
+// sums B_kj * dC_ij (deterministic).  A launch takes rows [0, MA) (row list
+// NULL) or the rows of a device list; rows whose C row passes NS entries are
+// appended to long_rows for the next tier, or set *too_long when there is no
+// next tier.
+template <typename V, typename RowT, int GS, int NS>
+__global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const int32_t *__restrict__ rows,
+                                                const unsigned int *__restrict__ nrows,
+                                                const RowT *__restrict__ arpt,
                                                 const int64_t *__restrict__ ab_b0,
                                                 const int32_t *__restrict__ ab_len,
                                                 const int32_t *__restrict__ bcol, const V *__restrict__ bval,
                                                 const RowT *__restrict__ crpt, const int32_t *__restrict__ ccol,
                                                 const V *__restrict__ dc, V *__restrict__ da,
+                                                const V *__restrict__ aval, const int64_t *__restrict__ poff,
+                                                V *__restrict__ w,
+                                                int32_t *__restrict__ long_rows, unsigned int *__restrict__ nlong,
                                                 uint32_t *__restrict__ too_long) {
   constexpr int RPB = 256 / GS;
   constexpr int SH = 2 * NS;
   __shared__ int32_t keys[RPB][SH];
   __shared__ V vals[RPB][SH];
-  __shared__ V acc[RPB][GS];
-  __shared__ EntryScratch<GS, V> ent[RPB];
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
   int32_t *K = keys[slot];
   V *W = vals[slot];
-  for (int64_t i = blockIdx.x * (int64_t)RPB + slot; i < MA; i += (int64_t)gridDim.x * RPB) {
+  const int64_t count = rows ? (int64_t)*nrows : MA;
+  for (int64_t t = blockIdx.x * (int64_t)RPB + slot; t < count; t += (int64_t)gridDim.x * RPB) {
+    const int64_t i = rows ? rows[t] : t;
     const int64_t c0 = (int64_t)crpt[i], nz = (int64_t)crpt[i + 1] - c0;
-    if (nz <= nz_lo) continue;
     if (nz > NS) {
-      if (too_long && glane == 0) *too_long = 1;
+      if (glane == 0) {
+        if (long_rows) long_rows[atomicAdd(nlong, 1u)] = (int32_t)i;
+        else *too_long = 1;
+      }
       continue;
     }
     const int log = table_log(nz);
@@ -231,31 +239,25 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, int64_t nz_lo, const
     }
     wave_lds_sync();
     const int64_t a0 = (int64_t)arpt[i], a1 = (int64_t)arpt[i + 1];
-    for (int64_t cc = a0; cc < a1; cc += GS) {
-      // one chunk of GS entries at a time so acc[] maps to the chunk
-      acc[slot][glane] = (V)0;
-      wave_lds_sync();
-      row_products<GS, R, false, V, true>(glane, cc, (cc + GS < a1 ? cc + GS : a1), 0, GS, ab_b0, ab_len, nullptr,
-                                          bcol, bval, &ent[slot],
-                                          [&](const int32_t(&key)[R], const V(&)[R], const bool(&ok)[R],
-                                              const V(&bv)[R], const int(&e)[R]) {
-#pragma unroll
-                                            for (int u = 0; u < R; u++)
-                                              if (ok[u])
-                                                __hip_atomic_fetch_add(&acc[slot][e[u]],
-                                                                       bv[u] * hash_lookup_rt<V>(K, W, key[u], log),
-                                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                          });
-      wave_lds_sync();
-      // map compacted entry index back to the entry's position
-      const int64_t ja = cc + glane;
-      int len = 0;
-      if (ja < a1) len = ab_len[ja];
-      const uint64_t nzm = group_ballot<GS>(len > 0);
-      const int cidx = __popcll(nzm & ((1ull << glane) - 1));
-      if (ja < a1) da[ja] = len > 0 ? acc[slot][cidx] : (V)0;
-      wave_lds_sync();
+    for (int64_t p = a0 + glane; p < a1; p += GS) {
+      const int64_t b0 = ab_b0[p];
+      const int len = ab_len[p];
+      V s = (V)0;
+      if (w) {
+        // also the product's share of dB: w[product] = A_ik dC_ij
+        const V a = aval[p];
+        V *wp = w + poff[p];
+        for (int j = 0; j < len; j++) {
+          const V d = hash_lookup_rt<V>(K, W, bcol[b0 + j], log);
+          s += bval[b0 + j] * d;
+          wp[j] = a * d;
+        }
+      } else {
+        for (int j = 0; j < len; j++) s += bval[b0 + j] * hash_lookup_rt<V>(K, W, bcol[b0 + j], log);
+      }
+      da[p] = s;
     }
+    wave_lds_sync();
   }
 }
 
@@ -267,6 +269,10 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, int64_t nz_lo, const
 // deterministic.  A's value of A^T entry t is AT.val[t], or A.val[perm[t]]
 // when the permutation is given.
 constexpr int kGradBRows = 64;
+
+// Largest per-product contribution buffer the backward allocates (bytes);
+// past it dB searches C's rows instead.
+constexpr size_t kBackwardContribMax = (size_t)8 << 30;
 
 template <typename V, typename RowT, int SG>
 __global__ __launch_bounds__(256) void k_grad_b(int64_t MB, const RowT *__restrict__ brpt,
@@ -318,6 +324,44 @@ __global__ __launch_bounds__(256) void k_grad_b(int64_t MB, const RowT *__restri
   }
 }
 
+// dB from the products' contributions w (written by k_grad_a, numbered in
+// the forward's product order: A entry p owns products poff[p] ..
+// poff[p] + nnz(B_k) - 1): dB_kq = sum over A^T row k of w[poff[perm[t]] + q].
+// Same chunking, lane split and fixed shuffle tree as k_grad_b.
+template <typename V, typename RowT, int SG>
+__global__ __launch_bounds__(256) void k_grad_b_sum(int64_t MB, const RowT *__restrict__ brpt,
+                                                    const RowT *__restrict__ atrpt, const RowT *__restrict__ at_perm,
+                                                    const int64_t *__restrict__ poff, const V *__restrict__ w,
+                                                    V *__restrict__ db) {
+  constexpr int NSG = 256 / SG;
+  __shared__ int64_t s_rpt[kGradBRows + 1];
+  const int sg = threadIdx.x / SG, sl = threadIdx.x % SG;
+  for (int64_t k0 = blockIdx.x * (int64_t)kGradBRows; k0 < MB; k0 += (int64_t)gridDim.x * kGradBRows) {
+    const int nr = (int)(MB - k0 < kGradBRows ? MB - k0 : kGradBRows);
+    for (int q = threadIdx.x; q <= nr; q += 256) s_rpt[q] = (int64_t)brpt[k0 + q];
+    __syncthreads();
+    const int64_t e0 = s_rpt[0], e1 = s_rpt[nr];
+    for (int64_t eb = e0; eb < e1; eb += NSG) {
+      const int64_t p = eb + sg;
+      const bool active = p < e1;
+      V s = (V)0;
+      if (active) {
+        int lo = 0, hi = nr;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_rpt[mid] <= p) lo = mid; else hi = mid;
+        }
+        const int64_t k = k0 + lo, q = p - s_rpt[lo];
+        const int64_t t0 = (int64_t)atrpt[k], t1 = (int64_t)atrpt[k + 1];
+        for (int64_t t = t0 + sl; t < t1; t += SG) s += w[poff[(int64_t)at_perm[t]] + q];
+      }
+      s = group_sum<SG>(s);
+      if (active && sl == 0) db[p] = s;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void k_zero_words(uint32_t *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
@@ -326,45 +370,89 @@ template <typename V, typename RowT>
 static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const nsp_csr_t *AT,
                          const RowT *at_perm, const nsp_csr_t *C, const V *dc, V *da, V *db) {
   hipStream_t st = h->stream;
-  // per-entry B row extents: the binning pass of the forward
+  // per-entry B row extents: the binning pass of the forward (reused when
+  // the handle allows it and the buffers are the same)
   NSP_TRY(spgemm_row_extents(h, sizeof(RowT) == 4 ? NSP_IDX32 : NSP_IDX64, A, B));
-  NSP_TRY(aux2_reserve(h, 256));
-  uint32_t *too_long = (uint32_t *)h->aux2.base;
-  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, too_long, 1);
-  const int64_t MA = A->rows;
+  const int64_t MA = A->rows, nnzA = A->nnz, nprod = h->ext_nprod;
+  // dB from per-product contributions when A^T's permutation is given and
+  // the contributions fit the budget, else by search in C's rows
+  const bool contrib = at_perm && nprod > 0 && (size_t)nprod * sizeof(V) <= kBackwardContribMax;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const int64_t ntiles = nnzA / kScanTile + 2;
+  const size_t o_rows = 256, o_status = o_rows + al((size_t)std::max<int64_t>(MA, 1) * 4),
+               o_poff = o_status + al((size_t)ntiles * 8), o_w = o_poff + al((size_t)(nnzA + 1) * 8),
+               need = contrib ? o_w + al((size_t)nprod * sizeof(V)) : o_poff;
+  if (need > h->bwd.bytes) {
+    h->poff_valid = false;
+    if (h->bwd.base) NSP_HIP(h, hipFree(h->bwd.base));
+    h->bwd.base = nullptr;
+    h->bwd.bytes = 0;
+    if (hipMalloc(&h->bwd.base, need + need / 8) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(h, NSP_ERR_OOM, "backward workspace allocation failed");
+    }
+    h->bwd.bytes = need + need / 8;
+  }
+  char *base = (char *)h->bwd.base;
+  uint32_t *flags = (uint32_t *)base;  // [0] too long, [1] rows for the second tier, [2] scan ticket
+  int32_t *long_rows = (int32_t *)(base + o_rows);
+  int64_t *poff = contrib ? (int64_t *)(base + o_poff) : nullptr;
+  V *w = contrib ? (V *)(base + o_w) : nullptr;
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 3);
+  if (contrib && !h->poff_valid) {
+    // product offsets of the A entries: exclusive scan of the B row lengths
+    uint64_t *status = (uint64_t *)(base + o_status);
+    hipLaunchKernelGGL(k_t_zero, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((ntiles + 255) / 256, 1024))),
+                       dim3(256), 0, st, (int64_t)0, (int64_t *)nullptr, status, ntiles, flags + 2);
+    hipLaunchKernelGGL((k_exscan<int32_t, int64_t>), dim3((unsigned)(nnzA / kScanTile + 1)), dim3(kScanBS), 0, st,
+                       nnzA, h->ab_len, poff, status, flags + 2);
+    h->poff_valid = true;
+  }
   if (MA > 0) {
-    // two tiers by C row length: <= 128 entries (32 lanes per row), 129..1024
-    // (64 lanes); each launch skips the other tier's rows
+    // tier 1: C rows up to 128 entries (32 lanes per row); longer rows go to
+    // a device list that tier 2 (up to 1024 entries, 64 lanes) walks
     const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 8));
-    hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128, 4>), dim3(ga), dim3(256), 0, st, MA, (int64_t)-1,
-                       (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col, (const V *)B->val,
-                       (const RowT *)C->row_ptr, C->col, dc, da, (uint32_t *)nullptr);
-    const int gb = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 3) / 4, (int64_t)h->num_cu * 4));
-    hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024, 4>), dim3(gb), dim3(256), 0, st, MA, (int64_t)128,
-                       (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col, (const V *)B->val,
-                       (const RowT *)C->row_ptr, C->col, dc, da, too_long);
+    hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128>), dim3(ga), dim3(256), 0, st, MA, (const int32_t *)nullptr,
+                       (const unsigned int *)nullptr, (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
+                       (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
+                       (const int64_t *)poff, w, long_rows, flags + 1, (uint32_t *)nullptr);
+    hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024>), dim3(h->num_cu), dim3(256), 0, st, MA,
+                       (const int32_t *)long_rows, (const unsigned int *)(flags + 1), (const RowT *)A->row_ptr,
+                       h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
+                       (const V *)A->val, (const int64_t *)poff, w, (int32_t *)nullptr, (unsigned int *)nullptr,
+                       flags);
   }
   if (B->rows > 0 && B->nnz > 0) {
     // lanes per B entry from the mean A^T row length
     const int64_t avg = A->cols > 0 ? (A->nnz + A->cols - 1) / A->cols : 1;
     const int g = (int)std::max<int64_t>(
         1, std::min<int64_t>((B->rows + kGradBRows - 1) / kGradBRows, (int64_t)h->num_cu * 8));
-#define GRAD_B(SG)                                                                                           \
-  hipLaunchKernelGGL((k_grad_b<V, RowT, SG>), dim3(g), dim3(256), 0, st, B->rows, (const RowT *)B->row_ptr,  \
-                     B->col, (const RowT *)AT->row_ptr, AT->col, (const V *)AT->val, (const V *)A->val, at_perm, \
-                     (const RowT *)C->row_ptr, C->col, dc, db)
-    if (avg <= 4) GRAD_B(4);
-    else if (avg <= 8) GRAD_B(8);
-    else if (avg <= 16) GRAD_B(16);
-    else if (avg <= 32) GRAD_B(32);
-    else GRAD_B(64);
+#define GRAD_B(SG)                                                                                            \
+  if (contrib)                                                                                                \
+    hipLaunchKernelGGL((k_grad_b_sum<V, RowT, SG>), dim3(g), dim3(256), 0, st, B->rows,                       \
+                       (const RowT *)B->row_ptr, (const RowT *)AT->row_ptr, at_perm, (const int64_t *)poff,   \
+                       (const V *)w, db);                                                                     \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_grad_b<V, RowT, SG>), dim3(g), dim3(256), 0, st, B->rows, (const RowT *)B->row_ptr, \
+                       B->col, (const RowT *)AT->row_ptr, AT->col, (const V *)AT->val, (const V *)A->val,     \
+                       at_perm, (const RowT *)C->row_ptr, C->col, dc, db)
+    if (avg <= 4) { GRAD_B(4); }
+    else if (avg <= 8) { GRAD_B(8); }
+    else if (avg <= 16) { GRAD_B(16); }
+    else if (avg <= 32) { GRAD_B(32); }
+    else { GRAD_B(64); }
 #undef GRAD_B
   }
   NSP_HIP(h, hipGetLastError());
+  // C's row lengths are checked once per C pattern: with fixed patterns
+  // (reuse_extents) later calls on the same C buffers need no host sync
+  if (h->reuse_extents && h->bwd_c_key[0] == C->row_ptr && h->bwd_c_key[1] == C->col) return NSP_OK;
   uint32_t tl = 0;
-  NSP_HIP(h, hipMemcpyAsync(&tl, too_long, 4, hipMemcpyDeviceToHost, st));
+  NSP_HIP(h, hipMemcpyAsync(&tl, flags, 4, hipMemcpyDeviceToHost, st));
   NSP_HIP(h, hipStreamSynchronize(st));
   if (tl) return set_error(h, NSP_ERR_UNSUPPORTED, "backward: C rows with more than 1024 entries");
+  h->bwd_c_key[0] = C->row_ptr;
+  h->bwd_c_key[1] = C->col;
   return NSP_OK;
 }
 

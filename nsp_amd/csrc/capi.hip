@@ -110,6 +110,7 @@ int nsp_destroy(nsp_handle_t hs) {
   if (h->ws.base) (void)hipFree(h->ws.base);
   if (h->aux.base) (void)hipFree(h->aux.base);
   if (h->aux2.base) (void)hipFree(h->aux2.base);
+  if (h->bwd.base) (void)hipFree(h->bwd.base);
   for (hipEvent_t e : h->events)
     if (e) (void)hipEventDestroy(e);
   if (h->d_info) (void)hipFree(h->d_info);
@@ -175,6 +176,8 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
     h->num_table_factor = (int)value;
   } else if (!std::strcmp(name, "timing")) {
     h->timing = value != 0;
+  } else if (!std::strcmp(name, "reuse_extents")) {
+    h->reuse_extents = value != 0;
   } else if (!std::strcmp(name, "heavy_blocks")) {
     if (value < 0 || value > 65536) return nsp::set_error(h, NSP_ERR_INVALID, "heavy_blocks must be in [0, 65536]");
     h->heavy_blocks = (int)value;

@@ -157,42 +157,6 @@ __device__ __forceinline__ T group_incl_scan(T x, int glane) {
 }
 
 // ---- LDS hash tables -------------------------------------------------------
-// Symbolic insert: returns 1 if key was new.  Linear probing; SH slots,
-// SH a power of two; keys holds kEmpty in unused slots.  Terminates whenever
-// the number of distinct keys inserted is <= SH.
-template <int LOG>
-__device__ __forceinline__ int hash_insert_key(int32_t* keys, int32_t key) {
-  constexpr uint32_t MASK = (1u << LOG) - 1;
-  uint32_t h = hash_slot<LOG>(key);
-  while (true) {
-    int32_t k = lds_ld(&keys[h]);
-    if (k == key) return 0;
-    if (k == kEmpty) {
-      int32_t old = atomicCAS(&keys[h], kEmpty, key);
-      if (old == kEmpty) return 1;
-      if (old == key) return 0;
-    }
-    h = (h + 1) & MASK;
-  }
-}
-
-// Numeric insert: accumulate v into key's slot (LDS atomic add).
-template <int LOG, typename V>
-__device__ __forceinline__ void hash_accumulate(int32_t* keys, V* vals, int32_t key, V v) {
-  constexpr uint32_t MASK = (1u << LOG) - 1;
-  uint32_t h = hash_slot<LOG>(key);
-  while (true) {
-    int32_t k = lds_ld(&keys[h]);
-    if (k == key) break;
-    if (k == kEmpty) {
-      int32_t old = atomicCAS(&keys[h], kEmpty, key);
-      if (old == kEmpty || old == key) break;
-    }
-    h = (h + 1) & MASK;
-  }
-  __hip_atomic_fetch_add(&vals[h], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
 // Value of a key known to be present.
 template <int LOG, typename V>
 __device__ __forceinline__ V hash_lookup(const int32_t* keys, const V* vals, int32_t key) {

@@ -66,6 +66,12 @@ struct Handle {
   int num_small_rows_per_wave = 4;
   int num_rows_per_wave = 2;
   int heavy_blocks = 0;  // 0: one block per CU
+  // the backward may reuse ab_b0/ab_len when they were computed for the same
+  // A and B buffers (the caller promises fixed patterns)
+  int reuse_extents = 0;
+  const void *ext_key[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t ext_rows = -1, ext_nnz = -1;
+  int64_t ext_nprod = -1;  // nprod of the cached extents (known once read back)
   int num_table_factor = 2;
   bool timing = false;
   bool timing_pending = false;
@@ -74,7 +80,10 @@ struct Handle {
   // reusable device workspace (ws: product; aux: compaction and gathers)
   Workspace ws;
   Workspace aux;
-  Workspace aux2;  // transpose and backward
+  Workspace aux2;  // transpose
+  Workspace bwd;   // backward: flags, row lists, product offsets, contributions
+  bool poff_valid = false;  // bwd holds the product offsets of the cached extents
+  const void *bwd_c_key[2] = {nullptr, nullptr};  // C pattern whose rows passed the length check
   Info *d_info = nullptr;
   Info *h_info = nullptr;  // pinned
 

@@ -124,19 +124,46 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
     }
     __syncthreads();
     const int64_t e0 = s_rpt[0], e1 = s_rpt[nr];
-    for (int64_t p = e0 + t; p < e1; p += 256) {
-      // row of entry p: the last q with s_rpt[q] <= p (empty rows skipped)
-      int lo = 0, hi = nr;
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (s_rpt[mid] <= p) lo = mid; else hi = mid;
+    // U independent entries per thread per round, so the A.col -> B.row_ptr
+    // gathers of several entries are in flight together.  Every thread runs
+    // every round (the wavefront shuffles below need all lanes).
+    constexpr int U = 4;
+    for (int64_t rb = e0; rb < e1; rb += U * 256) {
+      const int64_t pb = rb + t;
+      int64_t p[U], k[U], b0[U], b1[U];
+      int row[U];
+      bool v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        p[u] = pb + u * 256;
+        v[u] = p[u] < e1;
+        k[u] = v[u] ? acol[p[u]] : 0;
       }
-      const int64_t k = s_base[lo] + acol[p];
-      const int64_t b0 = (int64_t)brpt[k];
-      const int64_t len = (int64_t)brpt[k + 1] - b0;
-      ab_b0[p] = b0;
-      ab_len[p] = (int32_t)len;
-      atomicAdd(&s_ip[lo], (unsigned long long)len);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        // row of entry p: the last q with s_rpt[q] <= p (empty rows skipped)
+        int lo = 0, hi = nr;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (s_rpt[mid] <= p[u]) lo = mid; else hi = mid;
+        }
+        row[u] = lo;
+        k[u] += s_base[lo];
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        b0[u] = v[u] ? (int64_t)brpt[k[u]] : 0;
+        b1[u] = v[u] ? (int64_t)brpt[k[u] + 1] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t len = v[u] ? b1[u] - b0[u] : 0;
+        if (v[u]) {
+          ab_b0[p[u]] = b0[u];
+          ab_len[p[u]] = (int32_t)len;
+        }
+        if (v[u]) atomicAdd(&s_ip[row[u]], (unsigned long long)len);
+      }
     }
     __syncthreads();
     if (t < nr) {
@@ -406,6 +433,13 @@ __global__ __launch_bounds__(kScanBS) void k_scan(int64_t M, const int32_t *__re
 }
 
 // ---------------------------------------------------------------------------
+// Accumulator of the LDS value tables: FP64 for FP64 and FP32 values alike.
+// On gfx950 an LDS FP32 atomic add (ds_add_f32) runs about 20x slower than
+// the FP64 one (ds_add_f64), with or without address conflicts, so FP32
+// products (formed in FP32) are summed in FP64 and rounded once on output:
+// faster, and never less accurate than FP32 accumulation.
+using Acc = double;
+
 // Numeric, GS lanes per row, SH-slot LDS table (keys + values) per row, then
 // compaction of the occupied slots, a register bitonic sort of the row's
 // columns (GS lanes x E keys) and a lookup of each sorted column's value.
@@ -430,14 +464,14 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
   constexpr int SPL = SH / GS;    // table slots per lane in the compaction
   static_assert(E >= 1, "sort width below group size");
   __shared__ int32_t keys[RPB][SHD];
-  __shared__ V vals[RPB][SHD];
+  __shared__ Acc vals[RPB][SHD];
   __shared__ uint32_t stage[RPB][NS];
   __shared__ EntryScratch<GS, V> ent[RPB];
   int64_t count, off;
   bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
   int32_t *K = keys[slot];
-  V *W = vals[slot];
+  Acc *W = vals[slot];
   uint32_t *S = stage[slot];
   for (int64_t base = blockIdx.x * (int64_t)RPB; base < count; base += (int64_t)gridDim.x * RPB) {
     const int64_t t = base + slot;
@@ -453,7 +487,10 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
       row_products<GS, R, true, V>(glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, aval, bcol, bval,
                                    &ent[slot],
                                    [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
-                                     hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, glane);
+                                     Acc pa[R];
+#pragma unroll
+                                     for (int u = 0; u < R; u++) pa[u] = (Acc)prod[u];
+                                     hash_accumulate_batch<LOG, R, Acc>(K, W, key, pa, ok, glane);
                                    });
       wave_lds_sync();
       // compact occupied slots (strided: slot q*GS + glane) into S
@@ -481,7 +518,7 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
         const int g = glane * E + q;
         if (g < nz) {
           ccol[c0 + g] = (int32_t)v[q];
-          cval[c0 + g] = hash_lookup<LOG, V>(K, W, (int32_t)v[q]);
+          cval[c0 + g] = (V)hash_lookup<LOG, Acc>(K, W, (int32_t)v[q]);
         }
       }
       wave_lds_sync();
@@ -507,7 +544,7 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
   constexpr int LOG = Log2<SH>::value;
   constexpr int NW = BS / 64;
   __shared__ int32_t K[SH + 64];
-  __shared__ V W[SH + 64];
+  __shared__ Acc W[SH + 64];
   __shared__ uint32_t S[NS];
   __shared__ EntryScratch<64, V> ent[NW];
   __shared__ int scan_tmp[NW + 1];
@@ -527,7 +564,10 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
     row_products<64, R, true, V>(lane, arpt[r], arpt[r + 1], 64 * w, 64 * NW, ab_b0, ab_len, aval, bcol, bval,
                                  &ent[w],
                                  [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
-                                   hash_accumulate_batch<LOG, R, V>(K, W, key, prod, ok, lane);
+                                   Acc pa[R];
+#pragma unroll
+                                   for (int u = 0; u < R; u++) pa[u] = (Acc)prod[u];
+                                   hash_accumulate_batch<LOG, R, Acc>(K, W, key, pa, ok, lane);
                                  });
     __syncthreads();
     constexpr int SPT = SH / BS;
@@ -548,7 +588,7 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
     for (int g = threadIdx.x; g < nz; g += BS) {
       const int32_t key = (int32_t)S[g];
       ccol[c0 + g] = key;
-      cval[c0 + g] = hash_lookup<LOG, V>(K, W, key);
+      cval[c0 + g] = (V)hash_lookup<LOG, Acc>(K, W, key);
     }
     __syncthreads();
   }
@@ -930,6 +970,7 @@ static int ensure_workspace(Handle *h, int64_t MA, int64_t nnzA) {
   size_t need = al((MA + 1) * 8) * 3 + al(nnzA * 8 + 8) + al(nnzA * 4 + 4) + al(MA * 4) * 3 + al(MA + 1) +
                 al(ntiles * 8);
   if (need > h->ws.bytes) {
+    h->ext_rows = -1;  // the cached B row extents go with the old buffer
     if (h->ws.base) NSP_HIP(h, hipFree(h->ws.base));
     h->ws.base = nullptr;
     h->ws.bytes = 0;
@@ -991,6 +1032,15 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
                      A->col, (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->a_rpt64, h->ab_b0,
                      h->ab_len, h->intprod, h->row_nz, h->cls, h->d_info);
   NSP_HIP(h, hipGetLastError());
+  // what ab_b0 / ab_len now describe
+  h->ext_key[0] = A->col;
+  h->ext_key[1] = A->row_ptr;
+  h->ext_key[2] = B->row_ptr;
+  h->ext_key[3] = bt.a_off;
+  h->ext_rows = MA;
+  h->ext_nnz = A->nnz;
+  h->ext_nprod = -1;
+  h->poff_valid = false;
   return NSP_OK;
 }
 
@@ -1261,13 +1311,22 @@ int graph_destroy(GraphPlan *gp) {
   return NSP_OK;
 }
 
-int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B) {
+int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, bool *recomputed) {
   NSP_TRY(check_csr_args(h, "A", A));
   NSP_TRY(check_csr_args(h, "B", B));
   if (A->cols != B->rows) return set_error(h, NSP_ERR_INVALID, "A.cols != B.rows");
+  if (recomputed) *recomputed = false;
+  if (h->reuse_extents && h->ext_key[0] == A->col && h->ext_key[1] == A->row_ptr && h->ext_key[2] == B->row_ptr &&
+      h->ext_key[3] == nullptr && h->ext_rows == A->rows && h->ext_nnz == A->nnz && h->ext_nprod >= 0)
+    return NSP_OK;
   h->have_plan = false;
-  if (it == NSP_IDX32) return run_rowinfo<int32_t>(h, A, B);
-  return run_rowinfo<int64_t>(h, A, B);
+  h->poff_valid = false;
+  if (recomputed) *recomputed = true;
+  if (it == NSP_IDX32) NSP_TRY(run_rowinfo<int32_t>(h, A, B));
+  else NSP_TRY(run_rowinfo<int64_t>(h, A, B));
+  NSP_TRY(read_info(h));
+  h->ext_nprod = h->h_info->nprod;
+  return NSP_OK;
 }
 
 int spgemm_intprod(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, int64_t *intprod,

@@ -22,7 +22,9 @@ int graph_launch(GraphPlan *gp);
 int graph_result(GraphPlan *gp, int64_t *nnz);
 const char *graph_last_error(GraphPlan *gp);
 int graph_destroy(GraphPlan *gp);
-int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B);
+/* B row extents of every A entry (ab_b0, ab_len) and nprod; *recomputed
+ * tells whether the cached ones were replaced. */
+int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, bool *recomputed = nullptr);
 int transpose(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp_csr_t *XT);
 int spgemm_backward(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
                     const nsp_csr_t *AT, const void *at_perm, const nsp_csr_t *C, const void *dc, void *da,

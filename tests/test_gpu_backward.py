@@ -185,7 +185,9 @@ def test_backward_general_patterns(nsp, orc, case, dtype, use_perm):
     err_b = np.max(np.abs(db.cpu().numpy() - ref_db) / np.where(abs_db > 0, abs_db, 1))
     print(f"  componentwise error dA {err_a:.2e} dB {err_b:.2e}", flush=True)
     assert err_a <= TOL[dtype] and err_b <= TOL[dtype]
-    # deterministic: a second call gives the same bits
+    # deterministic: a second call gives the same bits, also when it reuses
+    # the first call's B row extents
+    h.set_param("reuse_extents", 1)
     da2, db2 = h.backward(dA_, dB_, AT, C, torch.from_numpy(dc).to(device="cuda", dtype=C.val.dtype),
                           at_perm=perm)
     assert torch.equal(da, da2) and torch.equal(db, db2)
