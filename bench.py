@@ -80,33 +80,70 @@ def measured_traffic(workload: str):
         return int(round(json.load(f)["traffic_bytes"]))
 
 
-def cpu_baseline(A, B, nprod_total, budget_s=20.0):
-    """Oracle Gustavson on this host.  Times the full product if a first
-    timed slice says it fits the budget, else a leading row block."""
+def cpu_baseline(A, B, nprod_total, budget_s=20.0, reps=5):
+    """The reference's host path on this machine's cores: the binary COO
+    reader/splitter (the oracle's restatement of READ_MATRIX_MPI.cpp and
+    distribute.cu) reading rank 0's row block of A and the P rows it needs,
+    then the oracle's row-wise Gustavson product over the same arrays.
+    Reader and product are timed separately, each the median of `reps` runs
+    after one warm-up.  If the whole product would pass the budget, a leading
+    block of rows is timed instead (stated in `sample`)."""
+    import statistics
+    import tempfile
     from oracle import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
     oracle.set_threads(threads)
     # calibrate on the first 1/32 of the rows
     r = max(1, A.nrows // 32)
-    rows = np.arange(r, dtype=np.int64)
     t0 = time.perf_counter()
-    oracle.spgemm_rows(A, B, rows, with_abs=False)
-    dt = time.perf_counter() - t0
-    ip, _ = oracle.intprod(A, B)
-    frac_time = dt * (A.nrows / r)
-    if frac_time * threads <= budget_s * threads:
+    oracle.spgemm_rows(A, B, np.arange(r, dtype=np.int64), with_abs=False)
+    frac_time = (time.perf_counter() - t0) * (A.nrows / r)
+    if frac_time * (reps + 1) <= budget_s:
         rows = np.arange(A.nrows, dtype=np.int64)
     else:
-        rows = np.arange(max(1, int(A.nrows * budget_s / frac_time)), dtype=np.int64)
+        rows = np.arange(max(1, int(A.nrows * budget_s / (frac_time * (reps + 1)))), dtype=np.int64)
+    ip, _ = oracle.intprod(A, B)
     nprod = int(ip[rows].sum())
-    t0 = time.perf_counter()
-    oracle.spgemm_rows(A, B, rows, with_abs=False)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        oracle.spgemm_rows(A, B, rows, with_abs=False)
+        times.append(time.perf_counter() - t0)
+    dt = statistics.median(times[1:])
+    # reader/splitter: B's rows as one file (rows 0.. of the matrix, header
+    # with the full row count 4*A.nrows), rank 0 of 4 reads A's block from it
+    read_s = None
+    try:
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "p.bin")
+            P = oracle_full_header(B, 4 * A.nrows)
+            oracle.write_binary_coo(path, P)
+            rt = []
+            for _ in range(reps + 1):
+                t0 = time.perf_counter()
+                oracle.read_binary_coo(path, 4, 0)
+                oracle.read_binary_coo(path, 1, 0)
+                rt.append(time.perf_counter() - t0)
+            read_s = statistics.median(rt[1:])
+    except OSError:
+        pass
     sample = (f"all {A.nrows} rows" if rows.size == A.nrows else f"rows [0, {rows.size}) of {A.nrows}")
     return {"value": round(2.0 * nprod / dt / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
-            "kind": "port", "seconds": round(dt, 3),
-            "sample": f"{sample} of the same product ({nprod} intermediate products), "
-                      f"row-wise Gustavson with a dense accumulator, OpenMP {threads} threads"}
+            "kind": "port", "seconds": round(dt, 4),
+            "reader_seconds": None if read_s is None else round(read_s, 4),
+            "hardware_concurrency": os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "sample": f"{sample} of the same product ({nprod} intermediate products): row-wise Gustavson "
+                      f"with a dense accumulator, OpenMP {threads} threads, median of {reps} runs; "
+                      f"reader_seconds = binary COO read of rank 0's A block plus the P rows "
+                      f"({A.nnz + B.nnz} records), median of {reps} runs"}
+
+
+def oracle_full_header(X, nrows_full):
+    """X with its row count raised to nrows_full (trailing rows empty), so
+    its file header states the whole matrix's size."""
+    from nsp_amd.gen import HostCSR
+    rpt = np.concatenate([X.rpt, np.full(nrows_full - X.nrows, X.rpt[-1], np.int64)])
+    return HostCSR(rpt, X.col, X.val, nrows_full, max(X.ncols, nrows_full))
 
 
 def _timed(fn, steps, warmup):
