@@ -34,7 +34,8 @@ struct Info {
   uint32_t tile_ticket;
   uint32_t error;
   int64_t cap;  // capacity of C's column/value buffers (numeric rows beyond it are skipped)
-  int64_t pad[3];
+  int64_t nlong;  // rows binned by the long-row pass
+  int64_t pad[2];
 };
 
 // A batch of independent products stored as concatenated rows: matrix b owns

@@ -53,149 +53,220 @@ __global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles, int64_
 }
 
 // ---------------------------------------------------------------------------
-// Row info: intermediate products per row (nsp.cu:306-320), symbolic bin,
-// per-bin histogram, nprod, an int64 copy of A's row pointer, and for every
-// A entry (i,k) the start and length of B row k (ab_b0, ab_len), which the
-// hash passes then read with one independent load instead of the dependent
-// A.col -> B.row_ptr gather.
+// Row info in two balanced passes.
 //
-// Entry-parallel: a block takes a chunk of kInfoRows consecutive rows, stages
-// their row pointers in LDS, and walks the chunk's entries one per thread
-// (coalesced A.col reads and ab_* writes, whatever the row lengths); each
-// entry finds its row by binary search over the staged pointers and adds its
-// B row length to the row's LDS counter.  One thread per row then finishes
-// the row.
+// k_extents (entry-parallel, every thread takes U entries per round): for
+// every A entry (i,k) the start and length of the B row it selects (ab_b0,
+// ab_len), which the hash passes then read with one independent load
+// instead of the dependent A.col -> B.row_ptr gather.  The row of an entry
+// is not needed, so the work is spread evenly whatever the row lengths
+// (power-law matrices put most entries in a few rows).
+//
+// k_rowinfo (16 lanes per row): intermediate products per row as the sum of
+// its entries' ab_len (nsp.cu:306-320), the symbolic bin, per-bin histogram,
+// nprod, max, and an int64 copy of A's row pointer.
 //
 // Batched products (a_row_off != NULL): A and B hold `count` matrices as
-// concatenated rows with matrix-local columns; row i of matrix b selects B
-// row b_row_off[b] + k for its column k.  This is the only place the batch
-// structure enters: every later pass works on the concatenated rows.
-constexpr int kInfoRows = 256;
+// concatenated rows with matrix-local columns; an entry of matrix b selects
+// B row b_row_off[b] + k.  A block finds the matrices its entries belong to
+// with one binary search over the matrices' first entries.  This is the only
+// place the batch structure enters: every later pass works on the
+// concatenated rows.
+constexpr int kExtU = 4;
+
+// Last b in [from, count) with arpt[a_row_off[b]] <= p, given that b = from
+// qualifies: gallop forward, then bisect (a block's tiles move forward
+// through the batch, so this is usually a step or two).
+template <typename RowT>
+__device__ __forceinline__ int64_t matrix_of_entry(const RowT *__restrict__ arpt, const int64_t *__restrict__ a_row_off,
+                                                   int64_t count, int64_t from, int64_t p) {
+  int64_t lo = from, hi = from + 1, step = 1;
+  while (hi < count && (int64_t)arpt[a_row_off[hi]] <= p) {
+    lo = hi;
+    step <<= 1;
+    hi = lo + step;
+  }
+  if (hi > count) hi = count;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)arpt[a_row_off[mid]] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
 
 template <typename RowT>
-__global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
+__global__ __launch_bounds__(256) void k_extents(int64_t nnzA, const RowT *__restrict__ arpt,
                                                  const int32_t *__restrict__ acol,
                                                  const RowT *__restrict__ brpt,
                                                  const int64_t *__restrict__ a_row_off,
                                                  const int64_t *__restrict__ b_row_off, int64_t count,
-                                                 int64_t *__restrict__ arpt64,
-                                                 int64_t *__restrict__ ab_b0,
-                                                 int32_t *__restrict__ ab_len,
-                                                 int64_t *__restrict__ intprod,
-                                                 int32_t *__restrict__ row_nz,
-                                                 uint8_t *__restrict__ cls, Info *info) {
-  __shared__ int64_t s_rpt[kInfoRows + 1];
-  __shared__ int64_t s_base[kInfoRows];
-  __shared__ unsigned long long s_ip[kInfoRows];
-  __shared__ unsigned long long hist[kBins];
-  __shared__ unsigned long long s_nprod, s_max;
+                                                 int64_t *__restrict__ ab_b0, int32_t *__restrict__ ab_len) {
   __shared__ int64_t s_mat[2];
   const int t = threadIdx.x;
-  if (t < kBins) hist[t] = 0;
-  if (t == 0) { s_nprod = 0; s_max = 0; }
-  unsigned long long my_sum = 0, my_max = 0;
-  for (int64_t r0 = blockIdx.x * (int64_t)kInfoRows; r0 < MA; r0 += (int64_t)gridDim.x * kInfoRows) {
-    const int nr = (int)(MA - r0 < kInfoRows ? MA - r0 : kInfoRows);
-    for (int q = t; q <= nr; q += 256) s_rpt[q] = (int64_t)arpt[r0 + q];
-    if (t < nr) s_ip[t] = 0;
-    if (a_row_off && t < 2) {
-      // matrices of the chunk's first and last rows: last b with a_row_off[b] <= i
-      const int64_t i = r0 + (t == 0 ? 0 : nr - 1);
-      int64_t lo = 0, hi = count;
-      while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (a_row_off[mid] <= i) lo = mid; else hi = mid;
+  constexpr int64_t TILE = 256 * kExtU;
+  // each block takes a contiguous run of tiles
+  const int64_t ntiles = (nnzA + TILE - 1) / TILE;
+  const int64_t t0 = blockIdx.x * ntiles / gridDim.x, t1 = (blockIdx.x + 1) * ntiles / gridDim.x;
+  int64_t from = 0;  // matrix search start (batched)
+  for (int64_t tile = t0; tile < t1; tile++) {
+    const int64_t tb = tile * TILE;
+    if (a_row_off) {
+      // matrices of the tile's first and last entries
+      if (t < 2) {
+        const int64_t p = t == 0 ? tb : (tb + TILE < nnzA ? tb + TILE : nnzA) - 1;
+        s_mat[t] = matrix_of_entry(arpt, a_row_off, count, from, p);
       }
-      s_mat[t] = lo;
+      __syncthreads();
     }
-    __syncthreads();
-    if (t < nr) {
-      int64_t bb = 0;
-      if (a_row_off) {
-        const int64_t i = r0 + t;
-        int64_t lo = s_mat[0], hi = s_mat[1] + 1;
+    int64_t p[kExtU], k[kExtU];
+    bool v[kExtU];
+#pragma unroll
+    for (int u = 0; u < kExtU; u++) {
+      p[u] = tb + u * 256 + t;
+      v[u] = p[u] < nnzA;
+      k[u] = v[u] ? acol[p[u]] : 0;
+    }
+    if (a_row_off) {
+      const int64_t m0 = s_mat[0], m1 = s_mat[1];
+#pragma unroll
+      for (int u = 0; u < kExtU; u++) {
+        int64_t lo = m0, hi = m1 + 1;
         while (hi - lo > 1) {
           const int64_t mid = (lo + hi) >> 1;
-          if (a_row_off[mid] <= i) lo = mid; else hi = mid;
+          if ((int64_t)arpt[a_row_off[mid]] <= p[u]) lo = mid; else hi = mid;
         }
-        bb = b_row_off[lo];
+        k[u] += b_row_off[lo];
       }
-      s_base[t] = bb;
+      from = m1;
     }
-    __syncthreads();
-    const int64_t e0 = s_rpt[0], e1 = s_rpt[nr];
-    // U independent entries per thread per round, so the A.col -> B.row_ptr
-    // gathers of several entries are in flight together.  Every thread runs
-    // every round (the wavefront shuffles below need all lanes).
-    constexpr int U = 4;
-    for (int64_t rb = e0; rb < e1; rb += U * 256) {
-      const int64_t pb = rb + t;
-      int64_t p[U], k[U], b0[U], b1[U];
-      int row[U];
-      bool v[U];
+    int64_t b0[kExtU], b1[kExtU];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        p[u] = pb + u * 256;
-        v[u] = p[u] < e1;
-        k[u] = v[u] ? acol[p[u]] : 0;
-      }
+    for (int u = 0; u < kExtU; u++) {
+      b0[u] = v[u] ? (int64_t)brpt[k[u]] : 0;
+      b1[u] = v[u] ? (int64_t)brpt[k[u] + 1] : 0;
+    }
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        // row of entry p: the last q with s_rpt[q] <= p (empty rows skipped)
-        int lo = 0, hi = nr;
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (s_rpt[mid] <= p[u]) lo = mid; else hi = mid;
-        }
-        row[u] = lo;
-        k[u] += s_base[lo];
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        b0[u] = v[u] ? (int64_t)brpt[k[u]] : 0;
-        b1[u] = v[u] ? (int64_t)brpt[k[u] + 1] : 0;
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int64_t len = v[u] ? b1[u] - b0[u] : 0;
-        if (v[u]) {
-          ab_b0[p[u]] = b0[u];
-          ab_len[p[u]] = (int32_t)len;
-        }
-        if (v[u]) atomicAdd(&s_ip[row[u]], (unsigned long long)len);
+    for (int u = 0; u < kExtU; u++) {
+      if (v[u]) {
+        ab_b0[p[u]] = b0[u];
+        ab_len[p[u]] = (int32_t)(b1[u] - b0[u]);
       }
     }
-    __syncthreads();
-    if (t < nr) {
-      const int64_t i = r0 + t;
-      const unsigned long long s = s_ip[t];
-      intprod[i] = (int64_t)s;
-      arpt64[i] = s_rpt[t];
-      if (s == 0) {
-        cls[i] = kNoBin;
-        row_nz[i] = 0;
-      } else {
-        const int b = sym_bin_of((int64_t)s);
-        cls[i] = (uint8_t)b;
-        atomicAdd(&hist[b], 1ull);
-      }
-      my_sum += s;
-      my_max = my_max > s ? my_max : s;
-    }
-    __syncthreads();  // the next chunk reuses s_rpt, s_ip, s_mat
+    if (a_row_off) __syncthreads();  // s_mat is rewritten by the next tile
   }
-  if (blockIdx.x == 0 && t == 0) arpt64[MA] = (int64_t)arpt[MA];
+}
+
+// rows with more entries than this are summed by a whole block
+constexpr int64_t kLongRow = 1024;
+
+// Finishes a row whose intermediate-product count is s.
+__device__ __forceinline__ void finish_row(int64_t i, unsigned long long s, int64_t p0, int64_t *__restrict__ intprod,
+                                           int64_t *__restrict__ arpt64, int32_t *__restrict__ row_nz,
+                                           uint8_t *__restrict__ cls, unsigned long long *hist) {
+  intprod[i] = (int64_t)s;
+  arpt64[i] = p0;
+  if (s == 0) {
+    cls[i] = kNoBin;
+    row_nz[i] = 0;
+  } else {
+    const int b = sym_bin_of((int64_t)s);
+    cls[i] = (uint8_t)b;
+    atomicAdd(&hist[b], 1ull);
+  }
+}
+
+template <typename RowT>
+__global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
+                                                 const int32_t *__restrict__ ab_len,
+                                                 int64_t *__restrict__ arpt64, int64_t *__restrict__ intprod,
+                                                 int32_t *__restrict__ row_nz, uint8_t *__restrict__ cls,
+                                                 int32_t *__restrict__ long_rows, Info *info) {
+  constexpr int GS = 16;
+  __shared__ unsigned long long hist[kBins];
+  __shared__ unsigned long long s_nprod, s_max;
+  if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
+  if (threadIdx.x == 0) { s_nprod = 0; s_max = 0; }
+  __syncthreads();
+  const int glane = threadIdx.x % GS;
+  unsigned long long my_sum = 0, my_max = 0;
+  for (int64_t i = blockIdx.x * (int64_t)(256 / GS) + threadIdx.x / GS; i < MA;
+       i += (int64_t)gridDim.x * (256 / GS)) {
+    const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
+    if (p1 - p0 > kLongRow) {  // k_rowinfo_long finishes it
+      if (glane == 0) long_rows[atomicAdd((unsigned long long *)&info->nlong, 1ull)] = (int32_t)i;
+      continue;
+    }
+    int64_t s = 0;
+    for (int64_t p = p0 + glane; p < p1; p += GS) s += ab_len[p];
+    s = group_sum<GS>(s);
+    if (glane == 0) {
+      finish_row(i, (unsigned long long)s, p0, intprod, arpt64, row_nz, cls, hist);
+      my_sum += (unsigned long long)s;
+      my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) arpt64[MA] = (int64_t)arpt[MA];
   my_sum = group_sum<64>(my_sum);
   my_max = group_max<64>(my_max);
-  if ((t & 63) == 0) {
+  if ((threadIdx.x & 63) == 0) {
     atomicAdd(&s_nprod, my_sum);
     atomicMax(&s_max, my_max);
   }
   __syncthreads();
-  if (t < kBins && hist[t]) atomicAdd((unsigned long long *)&info->count_sym[t], hist[t]);
-  if (t == 0) {
+  if (threadIdx.x < kBins && hist[threadIdx.x])
+    atomicAdd((unsigned long long *)&info->count_sym[threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x == 0) {
     atomicAdd((unsigned long long *)&info->nprod, s_nprod);
     atomicMax((unsigned long long *)&info->max_intprod, s_max);
+  }
+}
+
+// Rows listed by k_rowinfo as longer than kLongRow entries: one block per
+// row sums its entries' B row lengths (8 independent loads per thread per
+// round), then finishes it like k_rowinfo.
+template <typename RowT>
+__global__ __launch_bounds__(256) void k_rowinfo_long(const RowT *__restrict__ arpt,
+                                                      const int32_t *__restrict__ ab_len,
+                                                      const int32_t *__restrict__ long_rows,
+                                                      int64_t *__restrict__ arpt64, int64_t *__restrict__ intprod,
+                                                      int32_t *__restrict__ row_nz, uint8_t *__restrict__ cls,
+                                                      Info *info) {
+  __shared__ unsigned long long hist[kBins];
+  __shared__ unsigned long long s_red[4];
+  const int64_t nlong = info->nlong;
+  if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long my_sum = 0, my_max = 0;
+  for (int64_t q = blockIdx.x; q < nlong; q += gridDim.x) {
+    const int64_t i = long_rows[q];
+    const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
+    int64_t s = 0;
+    for (int64_t pb = p0; pb < p1; pb += 8 * 256) {
+      int32_t l[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int64_t p = pb + u * 256 + threadIdx.x;
+        l[u] = p < p1 ? ab_len[p] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += l[u];
+    }
+    s = group_sum<64>(s);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = (unsigned long long)s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const unsigned long long t = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+      finish_row(i, t, p0, intprod, arpt64, row_nz, cls, hist);
+      my_sum += t;
+      my_max = my_max > t ? my_max : t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kBins && hist[threadIdx.x])
+    atomicAdd((unsigned long long *)&info->count_sym[threadIdx.x], hist[threadIdx.x]);
+  if (threadIdx.x == 0 && my_sum) {
+    atomicAdd((unsigned long long *)&info->nprod, my_sum);
+    atomicMax((unsigned long long *)&info->max_intprod, my_max);
   }
 }
 
@@ -1026,11 +1097,19 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
   timing_mark(h, 0);
   hipLaunchKernelGGL(k_init, dim3(std::max<int64_t>(1, std::min<int64_t>((h->ntiles + 255) / 256, 1024))),
                      dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles, cap);
-  const int g =
-      (int)std::max<int64_t>(1, std::min<int64_t>((MA + kInfoRows - 1) / kInfoRows, (int64_t)h->num_cu * 8));
-  hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr,
-                     A->col, (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->a_rpt64, h->ab_b0,
-                     h->ab_len, h->intprod, h->row_nz, h->cls, h->d_info);
+  const int64_t nnzA = A->nnz;
+  if (nnzA > 0) {
+    const int ge = (int)std::max<int64_t>(
+        1, std::min<int64_t>((nnzA + 256 * kExtU - 1) / (256 * kExtU), (int64_t)h->num_cu * 16));
+    hipLaunchKernelGGL(k_extents<RowT>, dim3(ge), dim3(256), 0, h->stream, nnzA, (const RowT *)A->row_ptr, A->col,
+                       (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->ab_b0, h->ab_len);
+  }
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 15) / 16, (int64_t)h->num_cu * 8));
+  // perm_num is free until the numeric binning: it holds the long-row list
+  hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr, h->ab_len,
+                     h->a_rpt64, h->intprod, h->row_nz, h->cls, h->perm_num, h->d_info);
+  hipLaunchKernelGGL(k_rowinfo_long<RowT>, dim3(h->num_cu), dim3(256), 0, h->stream, (const RowT *)A->row_ptr,
+                     h->ab_len, (const int32_t *)h->perm_num, h->a_rpt64, h->intprod, h->row_nz, h->cls, h->d_info);
   NSP_HIP(h, hipGetLastError());
   // what ab_b0 / ab_len now describe
   h->ext_key[0] = A->col;
