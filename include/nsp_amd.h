@@ -13,7 +13,8 @@
  *   - row pointers int32 (NSP_IDX32, the reference's `int *rpt`,
  *     nsp/inc/nsp.h:57-70) or int64 (NSP_IDX64, for nnz >= 2^31); one width
  *     for A, B and C in a call.  Columns are always int32.
- *   - values FP64 (the reference's `real`) or FP32 with FP32 accumulation.
+ *   - values FP64 (the reference's `real`) or FP32; FP32 products are formed
+ *     in FP32 and summed in FP64 before the result is rounded to FP32.
  *   - C rows come out sorted ascending, structural zeros kept, as the
  *     reference produces them (nsp/src/nsp.cu:24-90, 1389-1399).
  *   - Functions return an nsp_status_t; they never exit the process (the
