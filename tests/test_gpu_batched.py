@@ -128,3 +128,49 @@ def test_graph_capacity_overflow_reported(nsp):
     with pytest.raises(nsp.NspError) as e:
         g.result()
     assert e.value.status == 3
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_batched_int64_rows_and_graph(nsp, orc, dtype):
+    """int64 row pointers through the batched path and the graph."""
+    print("case: batched int64", dtype, flush=True)
+    import torch
+    X, off = gen.uneven_batch_concat(20, seed=21, nmin=16, nmax=500)
+    X = to_dtype(X, dtype)
+    h = nsp.Handle()
+    dX = nsp.DeviceCSR.from_host(X, dtype, "i64")
+    d_off = dev_off(off)
+    ref = oracle_batch(orc, X, off)
+    C = h.batched_spgemm(dX, d_off, dX, d_off)
+    assert C.row_ptr.dtype == torch.int64
+    check_against_oracle(X, X, C, dtype, orc, ref=ref)
+    g = h.graph(dX, dX, capacity=int(ref[0][-1]), a_off=d_off, b_off=d_off)
+    for _ in range(2):
+        g.launch()
+        check_against_oracle(X, X, g.result(), dtype, orc, ref=ref)
+        torch.cuda.synchronize()
+
+
+def test_batched_with_empty_matrices_and_rows(nsp, orc):
+    """Matrices with no rows, and rows with no entries, inside a batch."""
+    print("case: batched empty members", flush=True)
+    rng = np.random.default_rng(31)
+    mats = []
+    for b in range(12):
+        n = 0 if b in (0, 5, 11) else int(rng.integers(8, 200))
+        m = gen.random_csr(n, n, 0.05, seed=100 + b, empty_rows=0.3) if n else \
+            gen.HostCSR(np.zeros(1, np.int64), np.zeros(0, np.int32), np.zeros(0), 0, 0)
+        mats.append(m)
+    off = np.zeros(len(mats) + 1, np.int64)
+    np.cumsum([m.nrows for m in mats], out=off[1:])
+    rpt = [np.zeros(1, np.int64)]
+    base = 0
+    for m in mats:
+        rpt.append(m.rpt[1:] + base)
+        base += m.nnz
+    X = gen.HostCSR(np.concatenate(rpt), np.concatenate([m.col for m in mats]).astype(np.int32),
+                    np.concatenate([m.val for m in mats]), int(off[-1]), max(m.ncols for m in mats))
+    h = nsp.Handle()
+    dX = nsp.DeviceCSR.from_host(X)
+    C = h.batched_spgemm(dX, dev_off(off), dX, dev_off(off))
+    check_against_oracle(X, X, C, "f64", orc, ref=oracle_batch(orc, X, off))

@@ -160,3 +160,24 @@ def test_rmat_scale20_sampled(nsp, orc):
         q = min(C.nnz, p + chunk)
         ok = (C.col[p:q] > C.col[p - 1:q - 1]) | start[p:q]
         assert bool(ok.all()), f"unsorted entries in [{p}, {q})"
+
+
+def test_heavy_rows_fp32_int64_graph(nsp, orc):
+    """Heavy rows with FP32 values and int64 row pointers, through the graph."""
+    print("case: heavy f32 i64 graph", flush=True)
+    import torch
+    from tests.helpers import to_dtype
+    rng = np.random.default_rng(41)
+    lens = rng.integers(1, 12, 300)
+    lens[[7, 150]] = [520, 610]
+    A = to_dtype(rows_matrix(lens, 4000, seed=42), "f32")
+    B = to_dtype(rows_matrix(rng.integers(40, 90, 4000), 90000, seed=43), "f32")
+    ref = orc.spgemm(A, B)
+    h = nsp.Handle()
+    g = h.graph(nsp.DeviceCSR.from_host(A, "f32", "i64"), nsp.DeviceCSR.from_host(B, "f32", "i64"),
+                capacity=int(ref[0][-1]))
+    for _ in range(2):
+        g.launch()
+        C = g.result()
+        assert C.row_ptr.dtype == torch.int64
+        check_against_oracle(A, B, C, "f32", orc, ref=ref)
