@@ -290,6 +290,68 @@ def rmat_leg(scale: int = 20, steps: int = 3, warmup: int = 1):
     return out
 
 
+def coo_leg(A, B, steps: int = 10, reps: int = 3):
+    """The reference's binary COO loader on the device: the file holds the
+    stencil's rows [0, n/4 + m^2) (header: the whole matrix's n rows) and rank
+    0 of 4 reads its share, rows [0, n/4).  Kernel: nsp_coo_to_csr on the
+    share's records already in HBM (24 bytes read and 12 written per record,
+    plus the row pointer).  End to end: nsp_coo_read from the page cache
+    (pinned chunks, copies overlapped with conversion) against the oracle's
+    CPU reader of the same share."""
+    import statistics
+    import tempfile
+    import torch
+    import nsp_amd
+    from oracle import oracle
+    h = nsp_amd.Handle()
+    h.set_param("timing", 1)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "p.bin")
+        oracle.write_binary_coo(path, oracle_full_header(B, 4 * A.nrows))
+        s = nsp_amd.coo_share(path, 4, 0)
+        nrec = s.rec_end - s.rec_begin
+        raw = np.fromfile(path, dtype=np.uint8, offset=s.header_bytes + 24 * s.rec_begin, count=24 * nrec)
+        rec = torch.from_numpy(raw).cuda()
+        del raw
+        X = h.coo_to_csr(rec, s.first_row, s.nrows, s.col_shift, ncols=s.nc)
+        t = []
+        for _ in range(steps):
+            h.coo_to_csr(rec, s.first_row, s.nrows, s.col_shift, ncols=s.nc)
+            t.append(h.stats()["t_coo_ms"])
+        kernel_ms = float(np.mean(t))
+        alg = 36 * nrec + 4 * (s.nrows + 1)
+        del rec, X
+        e2e = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, Xf = h.coo_read(path, 4, 0)
+            torch.cuda.synchronize()
+            e2e.append(time.perf_counter() - t0)
+        cpu = []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            _, _, _, shift, ref = oracle.read_binary_coo(path, 4, 0)
+            cpu.append(time.perf_counter() - t0)
+        equal = (shift == s.col_shift and np.array_equal(Xf.row_ptr.cpu().numpy(), ref.rpt)
+                 and np.array_equal(Xf.col.cpu().numpy(), ref.col) and np.array_equal(Xf.val.cpu().numpy(), ref.val))
+        file_bytes = 24 * nrec
+    e2e_s, cpu_s = statistics.median(e2e[1:]), statistics.median(cpu[1:])
+    achieved = alg / (kernel_ms / 1e3) / 1e9
+    return {"workload": f"binary COO share: rank 0 of 4 of the 27-point stencil file ({nrec} records, "
+                        f"{s.nrows} rows)",
+            "records": nrec, "rows": s.nrows,
+            "kernel_ms": round(kernel_ms, 4),
+            "roofline": {"bound": "hbm", "kernel": "k_coo_chunk (records -> CSR)", "achieved": round(achieved, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": alg},
+            "file_to_device_csr_s": round(e2e_s, 4),
+            "file_GBs": round(file_bytes / e2e_s / 1e9, 2),
+            "cpu_reader_s": round(cpu_s, 4), "cpu_reader_kind": "port (oracle, one thread)",
+            "speedup_vs_cpu_reader": round(cpu_s / e2e_s, 2),
+            "equal_to_cpu_reader": bool(equal)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,8 +359,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="fwd_bwd,batched,rmat",
-                    help="extra measurements after the headline: fwd_bwd, batched, rmat "
+    ap.add_argument("--legs", default="fwd_bwd,batched,rmat,coo",
+                    help="extra measurements after the headline: fwd_bwd, batched, rmat, coo "
                          "(comma list, or none)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's pick")
@@ -404,6 +466,9 @@ def main():
         torch.cuda.empty_cache()
     if "rmat" in legs:
         out["rmat"] = rmat_leg()
+    if "coo" in legs:
+        torch.cuda.empty_cache()
+        out["coo"] = coo_leg(A, B)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, B, nprod)
     print(json.dumps(out), flush=True)

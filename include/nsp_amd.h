@@ -74,6 +74,10 @@ typedef struct {
   /* device time of each phase (HIP events on the handle's stream), filled
    * when the "timing" parameter is 1; milliseconds */
   float t_bin_ms, t_sym_ms, t_scan_ms, t_num_ms;
+  /* device time of the record conversion of the last nsp_coo_to_csr (the
+   * conversion kernel alone) or nsp_coo_read (first chunk's conversion to
+   * the last one's end, copies included), when "timing" is 1 */
+  float t_coo_ms;
 } nsp_stats_t;
 
 /* ---- handle ------------------------------------------------------------ */
@@ -106,7 +110,9 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *                  row-pointer and C's buffers are the same (the caller
  *                  promises the patterns did not change); such calls do not
  *                  synchronise the host
- *   "timing"       1 records HIP events around each phase (see nsp_stats_t) */
+ *   "timing"       1 records HIP events around each phase (see nsp_stats_t)
+ *   "coo_chunk_records"  records per streamed chunk of nsp_coo_read
+ *                  (1..2^26, default 2^21: two 48 MiB pinned buffers) */
 int nsp_set_param(nsp_handle_t h, const char *name, int64_t value);
 
 /* ---- local product ----------------------------------------------------- */
@@ -192,8 +198,9 @@ int nsp_graph_destroy(nsp_graph_t g);
 /* Transpose of X into XT (XT->rows = X->cols, row pointer, columns and values
  * caller-allocated; XT rows come out sorted by X's row index).  Replaces the
  * reference's cuSPARSE csr2csc transpose CSRm::T (src/CSR.cu:739-797); cached
- * by the caller as A^T for nsp_spgemm_backward.  Columns of X with more than
- * 4096 entries return NSP_ERR_UNSUPPORTED. */
+ * by the caller as A^T for nsp_spgemm_backward.  Rows of XT up to 4096
+ * entries are sorted in LDS, longer ones by an LDS-tile + merge-path sort
+ * (one extra host synchronisation when there are any). */
 int nsp_transpose(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp_csr_t *XT);
 
 /* Gradients of C = A*B for a loss L(C) on fixed patterns, given dC = dL/dC on
@@ -232,6 +239,54 @@ int nsp_gather_rows_count(nsp_handle_t h, nsp_index_t it, const nsp_csr_t *B, in
  * out->row_ptr is the row pointer from nsp_gather_rows_count. */
 int nsp_gather_rows_fill(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *B, int64_t nr,
                          const int32_t *rows, int64_t row_offset, int64_t col_shift, nsp_csr_t *out);
+
+/* ---- binary COO files (the reference's on-disk matrix format) ----------- */
+/* File layout: an int64 header {nr, nc, nt} (header_type 1) or {nr, nt}
+ * (header_type 0, nc = nr), then nt records {int64 i, int64 j, double v},
+ * 1-based, sorted by row (src/READ_MATRIX_MPI.cpp:146-356 with Ti = long,
+ * the instantiation main.cu:164-165 uses).  Rank `rank` of `nprocs` owns
+ * rows [nr/nprocs*rank, nr/nprocs*(rank+1)), the last rank up to nr. */
+typedef struct {
+  int64_t nr, nc, nt;          /* the header */
+  int64_t header_bytes;        /* 24 or 16 */
+  int64_t rec_begin, rec_end;  /* the rank's records [rec_begin, rec_end) */
+  int64_t first_row;           /* 0-based row of the rank's first record */
+  int64_t nrows;               /* rows spanned, first to last present row (rowxproc) */
+  int64_t col_shift;           /* sum of nrows over the ranks before this one */
+} nsp_coo_share_t;
+
+/* Host only (no device work): reads the header and locates the rank's
+ * records by binary search on the row id, and the col_shift that the
+ * reference derives with MPI_Exscan over every rank's rowxproc.  Replaces the
+ * header read and binsearch_iat of READ_BINARY_MATRIX_MPI
+ * (src/READ_MATRIX_MPI.cpp:83-130, 230-270) and the rowxproc / col_shift
+ * computation of load_MatrixMPI (src/distribute.cu:1094-1095).  A rank with
+ * no records gets nrows 0 (the reference reads irow[-1] there). */
+int nsp_coo_share(const char *path, int header_type, int nprocs, int rank, nsp_coo_share_t *out);
+
+/* CSR of `nrec` records already in device memory (24-byte records as in the
+ * file), rows first_row .. first_row+out->rows-1 (0-based): out->row_ptr
+ * (out->rows+1 entries, width it), out->col = j-1-col_shift (int32),
+ * out->val = v (FP64, or rounded to FP32 with NSP_F32), all caller-allocated
+ * device memory of out->nnz = nrec entries.  Rows whose columns are out of
+ * order are sorted stably, as check_and_fix_order's bubble sort does.
+ * Records out of row order or outside the rows return NSP_ERR_INVALID; a
+ * column that does not fit int32 returns NSP_ERR_OVERFLOW.  Replaces the
+ * record unpacking of READ_BINARY_MATRIX_MPI (src/READ_MATRIX_MPI.cpp:
+ * 329-341), irow2iat2, check_and_fix_order and apply_shift
+ * (src/distribute.cu:998-1069, 1097-1105).  Synchronises the host once. */
+int nsp_coo_to_csr(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const void *records, int64_t nrec,
+                   int64_t first_row, int64_t col_shift, nsp_csr_t *out);
+
+/* The rank's share of a binary COO file straight into a device CSR:
+ * records stream from the file through pinned staging buffers and are
+ * converted on the device chunk by chunk, reads overlapping copies and
+ * conversion.  out->row_ptr (share->nrows+1), out->col and out->val
+ * (share->rec_end - share->rec_begin entries) are caller-allocated device
+ * memory; out->rows/cols/nnz are set.  Errors as nsp_coo_to_csr.  Replaces
+ * load_MatrixMPI (src/distribute.cu:1071-1125) for one rank. */
+int nsp_coo_read(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const char *path, int header_type,
+                 const nsp_coo_share_t *share, nsp_csr_t *out);
 
 #ifdef __cplusplus
 }

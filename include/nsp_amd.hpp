@@ -13,7 +13,8 @@
  *                                     full P on the same device
  *   read_binary_matrix                src/READ_MATRIX_MPI.cpp:146-356 with the
  *                                     row split and column shift of
- *                                     src/distribute.cu:1071-1125
+ *                                     src/distribute.cu:1071-1125 (host CSR)
+ *   load_MatrixMPI                    src/distribute.cu:1071-1125 (device CSR)
  *   write_binary_matrix               main.cu:468-512
  * Errors: where the reference exits (checkCudaErrors, helper_cuda.h:714) these
  * functions throw nspamd::Error carrying the C ABI status.  Calls are
@@ -140,11 +141,18 @@ CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local
 
 /* Rank `rank` of `nprocs` of a binary COO matrix file (header {nr, nc, nt}
  * when header_type = 1, {nr, nt} when 0, then nt records {int64 i, int64 j,
- * double v}, 1-based, sorted by row): host CSR with n = rows present in the
- * rank's range, columns shifted by the block's first row (row_shift), rows
- * sorted by column.  *nr_out/*nc_out/*nt_out get the file's sizes. */
+ * double v}, 1-based, sorted by row): host CSR whose n rows run from the
+ * first to the last row present in the rank's range, columns shifted by
+ * row_shift = the rows of the ranks before (the reference's MPI_Exscan of
+ * rowxproc; the block's first row unless empty rows sit at partition ends),
+ * rows sorted by column.  *nr_out/*nc_out/*nt_out get the file's sizes. */
 CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type = 1, int64_t *nr_out = nullptr,
                         int64_t *nc_out = nullptr, int64_t *nt_out = nullptr);
+/* The same block as a device CSR built on the device (nsp_coo_read: records
+ * stream from the file through pinned buffers and are converted there), as
+ * the reference's load_MatrixMPI returns it (src/distribute.cu:1071-1125,
+ * binary files, row partition). */
+CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type = 1);
 /* Writes a host CSR as binary COO (header_type 1); global column =
  * col + col_shift. */
 void write_binary_matrix(const char *path, const CSR *A, int64_t col_shift = 0);

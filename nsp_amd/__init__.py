@@ -112,7 +112,8 @@ class Handle:
                 "num_bin": list(st.num_bin[:st.n_num_bins]), "direct_mode": st.direct_mode,
                 "host_syncs": st.host_syncs, "sym_launches": st.sym_launches,
                 "num_launches": st.num_launches, "t_bin_ms": st.t_bin_ms,
-                "t_sym_ms": st.t_sym_ms, "t_scan_ms": st.t_scan_ms, "t_num_ms": st.t_num_ms}
+                "t_sym_ms": st.t_sym_ms, "t_scan_ms": st.t_scan_ms, "t_num_ms": st.t_num_ms,
+                "t_coo_ms": st.t_coo_ms}
 
     # -- local product ---------------------------------------------------
     def symbolic(self, A: DeviceCSR, B: DeviceCSR):
@@ -241,6 +242,53 @@ class Handle:
                                                 C.c_void_p(rows.data_ptr() if nr else None),
                                                 int(row_offset), int(col_shift), C.byref(o)))
         return out
+
+    # -- binary COO files ---------------------------------------------------
+    def coo_to_csr(self, records, first_row: int, nrows: int, col_shift: int = 0, ncols: int = 0,
+                   index: str = "i32", dtype: str = "f64") -> DeviceCSR:
+        """CSR of 24-byte {int64 i, int64 j, double v} records (a uint8 or
+        int64 device tensor holding them), rows first_row .. first_row +
+        nrows - 1, columns j - 1 - col_shift (nsp_coo_to_csr)."""
+        torch = _torch()
+        nrec = records.numel() * records.element_size() // 24
+        dev = records.device
+        out = DeviceCSR(torch.empty(nrows + 1, dtype=torch.int64 if index == "i64" else torch.int32, device=dev),
+                        torch.empty(nrec, dtype=torch.int32, device=dev),
+                        torch.empty(nrec, dtype=torch.float32 if dtype == "f32" else torch.float64, device=dev),
+                        int(nrows), int(ncols))
+        o = out.c_struct()
+        self._check(self.L.nsp_coo_to_csr(self.h, out.index_type, out.value_type,
+                                          C.c_void_p(records.data_ptr() if nrec else None), nrec,
+                                          int(first_row), int(col_shift), C.byref(o)))
+        return out
+
+    def coo_read(self, path: str, nprocs: int = 1, rank: int = 0, header_type: int = 1,
+                 index: str = "i32", dtype: str = "f64", share=None):
+        """(share, DeviceCSR): rank `rank` of `nprocs` of a binary COO file
+        built on the device (nsp_coo_read); share as coo_share returns it."""
+        torch = _torch()
+        s = share if share is not None else coo_share(path, nprocs, rank, header_type)
+        nrec = s.rec_end - s.rec_begin
+        dev = torch.device("cuda", torch.cuda.current_device())
+        out = DeviceCSR(torch.empty(s.nrows + 1, dtype=torch.int64 if index == "i64" else torch.int32,
+                                    device=dev),
+                        torch.empty(nrec, dtype=torch.int32, device=dev),
+                        torch.empty(nrec, dtype=torch.float32 if dtype == "f32" else torch.float64, device=dev),
+                        int(s.nrows), int(s.nc))
+        o = out.c_struct()
+        self._check(self.L.nsp_coo_read(self.h, out.index_type, out.value_type, path.encode(), int(header_type),
+                                        C.byref(s), C.byref(o)))
+        return s, out
+
+
+def coo_share(path: str, nprocs: int = 1, rank: int = 0, header_type: int = 1) -> "_lib.nsp_coo_share_t":
+    """Header and rank range of a binary COO file (nsp_coo_share; host
+    only, no GPU needed)."""
+    s = _lib.nsp_coo_share_t()
+    st = _lib.load().nsp_coo_share(path.encode(), int(header_type), int(nprocs), int(rank), C.byref(s))
+    if st != _lib.NSP_OK:
+        raise NspError(st, f"cannot read the header or rows of {path}")
+    return s
 
 
 class Graph:

@@ -29,7 +29,13 @@ class nsp_stats_t(C.Structure):
                 ("n_num_bins", C.c_int32), ("direct_mode", C.c_int32), ("host_syncs", C.c_int32),
                 ("sym_launches", C.c_int32), ("num_launches", C.c_int32),
                 ("t_bin_ms", C.c_float), ("t_sym_ms", C.c_float), ("t_scan_ms", C.c_float),
-                ("t_num_ms", C.c_float)]
+                ("t_num_ms", C.c_float), ("t_coo_ms", C.c_float)]
+
+
+class nsp_coo_share_t(C.Structure):
+    _fields_ = [("nr", C.c_int64), ("nc", C.c_int64), ("nt", C.c_int64), ("header_bytes", C.c_int64),
+                ("rec_begin", C.c_int64), ("rec_end", C.c_int64), ("first_row", C.c_int64),
+                ("nrows", C.c_int64), ("col_shift", C.c_int64)]
 
 
 class NspError(RuntimeError):
@@ -68,6 +74,9 @@ SIGNATURES = [
     ("nsp_graph_destroy", _I, [_H]),
     ("nsp_transpose", _I, [_H, _I, _I, _CSRP, _CSRP]),
     ("nsp_spgemm_backward", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _VP, _CSRP, _VP, _VP, _VP]),
+    ("nsp_coo_share", _I, [C.c_char_p, _I, _I, _I, C.POINTER(nsp_coo_share_t)]),
+    ("nsp_coo_to_csr", _I, [_H, _I, _I, _VP, _I64, _I64, _I64, _CSRP]),
+    ("nsp_coo_read", _I, [_H, _I, _I, C.c_char_p, _I, C.POINTER(nsp_coo_share_t), _CSRP]),
 ]
 
 

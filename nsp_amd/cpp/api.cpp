@@ -292,58 +292,46 @@ struct CooRec {
   double v;
 };
 
-// first record whose row is >= row1 (1-based rows, file sorted by row)
-int64_t lower_bound_row(FILE *f, int64_t hdr, int64_t nt, int64_t row1) {
-  int64_t lo = 0, hi = nt;
-  while (lo < hi) {
-    const int64_t mid = lo + (hi - lo) / 2;
-    CooRec r;
-    if (fseeko(f, (off_t)(hdr + mid * (int64_t)sizeof(CooRec)), SEEK_SET) != 0 || fread(&r, sizeof r, 1, f) != 1)
-      return -1;
-    if (r.i < row1) lo = mid + 1; else hi = mid;
-  }
-  return lo;
+nsp_coo_share_t coo_share(const char *path, int nprocs, int rank, int header_type, const char *who) {
+  if (nprocs < 1 || rank < 0 || rank >= nprocs) throw Error(NSP_ERR_INVALID, std::string(who) + ": bad rank");
+  nsp_coo_share_t s;
+  const int st = nsp_coo_share(path, header_type, nprocs, rank, &s);
+  if (st != NSP_OK) throw Error(st, std::string(who) + ": cannot read the header or rows of " + path);
+  if (s.rec_end - s.rec_begin > INT32_MAX || s.nrows > INT32_MAX)
+    throw Error(NSP_ERR_OVERFLOW, std::string(who) + ": the rank's share does not fit int32 CSR");
+  return s;
 }
 
 }  // namespace
 
 CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type, int64_t *nr_out, int64_t *nc_out,
                         int64_t *nt_out) {
-  if (nprocs < 1 || rank < 0 || rank >= nprocs) throw Error(NSP_ERR_INVALID, "read_binary_matrix: bad rank");
+  const nsp_coo_share_t s = coo_share(path, nprocs, rank, header_type, "read_binary_matrix");
+  const int64_t nrec = s.rec_end - s.rec_begin;
+  std::vector<CooRec> rec((size_t)nrec);
   FILE *f = fopen(path, "rb");
   if (!f) throw Error(NSP_ERR_INVALID, std::string("read_binary_matrix: cannot open ") + path);
-  int64_t h[3] = {0, 0, 0};
-  const int nh = header_type ? 3 : 2;
-  if (fread(h, 8, nh, f) != (size_t)nh) {
-    fclose(f);
-    throw Error(NSP_ERR_INVALID, "read_binary_matrix: short header");
-  }
-  const int64_t nr = h[0], nc = header_type ? h[1] : h[0], nt = header_type ? h[2] : h[1];
-  const int64_t hdr = 8 * nh;
-  // the rank's rows (READ_MATRIX_MPI.cpp:256-270): nr/nprocs each, the rest to the last
-  const int64_t per = nr / nprocs, first = per * rank, last = rank == nprocs - 1 ? nr : per * (rank + 1);
-  const int64_t b = lower_bound_row(f, hdr, nt, first + 1), e = lower_bound_row(f, hdr, nt, last + 1);
-  if (b < 0 || e < 0) {
-    fclose(f);
-    throw Error(NSP_ERR_INVALID, "read_binary_matrix: truncated file");
-  }
-  std::vector<CooRec> rec((size_t)(e - b));
-  if (!rec.empty() && (fseeko(f, (off_t)(hdr + b * (int64_t)sizeof(CooRec)), SEEK_SET) != 0 ||
-                       fread(rec.data(), sizeof(CooRec), rec.size(), f) != rec.size())) {
-    fclose(f);
-    throw Error(NSP_ERR_INVALID, "read_binary_matrix: truncated records");
-  }
+  const bool ok = rec.empty() || (fseeko(f, (off_t)(s.header_bytes + s.rec_begin * (int64_t)sizeof(CooRec)),
+                                         SEEK_SET) == 0 &&
+                                  fread(rec.data(), sizeof(CooRec), rec.size(), f) == rec.size());
   fclose(f);
-  // the block spans the rows it holds (distribute.cu:1096): first to last row present
-  const int64_t r_first = rec.empty() ? first : rec.front().i - 1;
-  const int64_t r_last = rec.empty() ? first - 1 : rec.back().i - 1;
-  const int n = (int)(r_last - r_first + 1), nnz = (int)rec.size();
-  CSR *A = CSRm::init(n, nc, nnz, true, false, nr, r_first);
+  if (!ok) throw Error(NSP_ERR_INVALID, "read_binary_matrix: truncated records");
+  // rows first to last present row (distribute.cu:1094), columns shifted by
+  // the rows of the ranks before (distribute.cu:1095, 1103)
+  const int n = (int)s.nrows, nnz = (int)nrec;
+  CSR *A = CSRm::init(n, s.nc, nnz, true, false, s.nr, s.col_shift);
   std::fill(A->row, A->row + n + 1, 0);
-  for (const CooRec &r : rec) A->row[r.i - 1 - r_first + 1]++;
+  for (const CooRec &r : rec) {
+    const int64_t x = r.i - 1 - s.first_row;
+    if (x < 0 || x >= n) {
+      CSRm::free(A);
+      throw Error(NSP_ERR_INVALID, "read_binary_matrix: records out of row order");
+    }
+    A->row[x + 1]++;
+  }
   for (int i = 0; i < n; i++) A->row[i + 1] += A->row[i];
   for (int q = 0; q < nnz; q++) {
-    A->col[q] = (int)(rec[q].j - 1 - r_first);  // rank frame (distribute.cu:1097-1105)
+    A->col[q] = (int)(rec[q].j - 1 - s.col_shift);
     A->val[q] = rec[q].v;
   }
   // rows in column order (check_and_fix_order): a stable sort per row
@@ -361,9 +349,23 @@ CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type,
       A->val[p] = tmp[p - p0].second;
     }
   }
-  if (nr_out) *nr_out = nr;
-  if (nc_out) *nc_out = nc;
-  if (nt_out) *nt_out = nt;
+  if (nr_out) *nr_out = s.nr;
+  if (nc_out) *nc_out = s.nc;
+  if (nt_out) *nt_out = s.nt;
+  return A;
+}
+
+CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type) {
+  const nsp_coo_share_t s = coo_share(path, nprocs, rank, header_type, "load_MatrixMPI");
+  const int n = (int)s.nrows, nnz = (int)(s.rec_end - s.rec_begin);
+  CSR *A = CSRm::init(n, s.nc, nnz, true, true, s.nr, s.col_shift);
+  nsp_handle_t h = handle();
+  nsp_csr_t out{n, s.nc, nnz, A->row, A->col, A->val};
+  const int st = nsp_coo_read(h, NSP_IDX32, NSP_F64, path, header_type, &s, &out);
+  if (st != NSP_OK) {
+    CSRm::free(A);
+    check(st, h, "load_MatrixMPI");
+  }
   return A;
 }
 

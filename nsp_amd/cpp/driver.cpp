@@ -10,6 +10,9 @@
 //   nsp_amd_run split A.bin nprocs rank out.bin
 //       the reader/splitter alone (host only): the rank's block, columns in
 //       the rank frame, written back as binary COO
+//   nsp_amd_run load A.bin nprocs rank out.bin
+//       the same block built on the device (load_MatrixMPI), copied back
+//       and written
 //
 // Matrices are binary COO files (header {nr, nc, nt}, then {i, j, v}
 // records, 1-based, sorted by row).  Prints one JSON line; C.bin, when given,
@@ -77,38 +80,56 @@ static int run_product(int argc, char **argv) {
 static int run_local(int argc, char **argv) {
   if (argc < 6) return 2;
   const int nprocs = atoi(argv[4]), rank = atoi(argv[5]);
-  int64_t nr = 0;
-  CSR *A = read_binary_matrix(argv[2], nprocs, rank, 1, &nr);
-  CSR *Pblk = read_binary_matrix(argv[3], nprocs, rank);
-  CSR *P = read_binary_matrix(argv[3], 1, 0);
-  CSR *Ad = CSRm::copyToDevice(A), *Pd = CSRm::copyToDevice(P);
+  // main.cu:164-165: both matrices through load_MatrixMPI (device CSR); P
+  // whole, since its halo rows are gathered on this device
+  const auto t0 = std::chrono::steady_clock::now();
+  CSR *Ad = load_MatrixMPI(argv[2], nprocs, rank);
+  CSR *Pd = load_MatrixMPI(argv[3], 1, 0);
+  const double read_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  nsp_coo_share_t pshare;
+  if (nsp_coo_share(argv[3], 1, nprocs, rank, &pshare) != NSP_OK) throw Error(NSP_ERR_INVALID, "P share");
   double loc_time = 0.0;
   CSR *A_ = nullptr, *P_ = nullptr;
-  CSR *C = nsparseMGPU_version1(Ad, Pd, loc_time, Pblk->n, &A_, &P_);
+  CSR *C = nsparseMGPU_version1(Ad, Pd, loc_time, (int)pshare.nrows, &A_, &P_);
   const double l2 = CSRm::norm_fro(C);
   if (argc > 6) write_device_csr(argv[6], C);
   printf("{\"mode\": \"local\", \"nprocs\": %d, \"rank\": %d, \"row_shift\": %lld, \"rows\": %d, "
          "\"shrinked_m_A\": %d, \"completedP_rows\": %d, \"shrinked_m_P\": %d, \"nnz\": %d, "
-         "\"l2norm\": %.17g, \"loc_time\": %.6f}\n",
-         nprocs, rank, (long long)A->row_shift, C->n, A_->shrinked_m ? A_->shrinked_m : (int)A_->m, P_->n,
-         (int)P_->m, C->nnz, l2, loc_time);
+         "\"l2norm\": %.17g, \"loc_time\": %.6f, \"read_s\": %.6f}\n",
+         nprocs, rank, (long long)Ad->row_shift, C->n, A_->shrinked_m ? A_->shrinked_m : (int)A_->m, P_->n,
+         (int)P_->m, C->nnz, l2, loc_time, read_s);
   CSRm::free(C);
   CSRm::free(P_);
   CSRm::freeStruct(A_);
   CSRm::free(Ad);
   CSRm::free(Pd);
-  CSRm::free(A);
-  CSRm::free(P);
-  CSRm::free(Pblk);
   return 0;
 }
 
-static int run_split(int argc, char **argv) {
+static int run_split(int argc, char **argv, bool device) {
   if (argc < 6) return 2;
   const int nprocs = atoi(argv[3]), rank = atoi(argv[4]);
   int64_t nr = 0, nc = 0, nt = 0;
   const auto t0 = std::chrono::steady_clock::now();
-  CSR *A = read_binary_matrix(argv[2], nprocs, rank, 1, &nr, &nc, &nt);
+  CSR *A = nullptr;
+  if (device) {
+    CSR *Ad = load_MatrixMPI(argv[2], nprocs, rank);
+    const double td = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    A = CSRm::copyToHost(Ad);
+    CSRm::free(Ad);
+    nsp_coo_share_t s;
+    if (nsp_coo_share(argv[2], 1, nprocs, rank, &s) != NSP_OK) throw Error(NSP_ERR_INVALID, "share");
+    nr = s.nr;
+    nc = s.nc;
+    nt = s.nt;
+    write_binary_matrix(argv[5], A, 0);
+    printf("{\"mode\": \"load\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %d, \"nnz\": %d, "
+           "\"row_shift\": %lld, \"read_s\": %.6f}\n",
+           (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, (long long)A->row_shift, td);
+    CSRm::free(A);
+    return 0;
+  }
+  A = read_binary_matrix(argv[2], nprocs, rank, 1, &nr, &nc, &nt);
   const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   write_binary_matrix(argv[5], A, 0);
   printf("{\"mode\": \"split\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %d, \"nnz\": %d, "
@@ -120,14 +141,18 @@ static int run_split(int argc, char **argv) {
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: %s product A.bin B.bin [C.bin] | local A.bin P.bin nprocs rank [C.bin]\n", argv[0]);
+    fprintf(stderr,
+            "usage: %s product A.bin B.bin [C.bin] | local A.bin P.bin nprocs rank [C.bin] | "
+            "split|load A.bin nprocs rank out.bin\n",
+            argv[0]);
     return 2;
   }
   try {
     int rc = 2;
     if (!strcmp(argv[1], "product")) rc = run_product(argc, argv);
     else if (!strcmp(argv[1], "local")) rc = run_local(argc, argv);
-    else if (!strcmp(argv[1], "split")) rc = run_split(argc, argv);
+    else if (!strcmp(argv[1], "split")) rc = run_split(argc, argv, false);
+    else if (!strcmp(argv[1], "load")) rc = run_split(argc, argv, true);
     if (rc == 2) fprintf(stderr, "bad arguments\n");
     return rc;
   } catch (const Error &e) {

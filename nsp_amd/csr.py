@@ -11,8 +11,12 @@ tensors.  The entry points mirror src/spspmpi.cu:
       MPI_Alltoallv, then A and completedP are compacted and multiplied.
 
 Row partition of a rank (src/READ_MATRIX_MPI.cpp:256-270): nr/nprocs rows
-per rank, the remainder to the last rank; a rank's columns are shifted by its
-first row (src/distribute.cu:1097-1105).
+per rank, the remainder to the last rank; a rank's rows run from its first
+to its last present row and its columns are shifted by the rows of the ranks
+before it (src/distribute.cu:1094-1105).
+
+  load_MatrixMPI(path, nprocs, rank)   distribute.cu:1071-1125 (binary files,
+      row partition): the rank's block built on the device.
 """
 from __future__ import annotations
 
@@ -47,6 +51,15 @@ class CSR:
     def from_device(cls, X: DeviceCSR, full_n: int | None = None, row_shift: int = 0) -> "CSR":
         return cls(X.nnz, X.nrows, X.ncols, X.nrows if full_n is None else full_n, row_shift,
                    X.row_ptr, X.col, X.val)
+
+
+def load_MatrixMPI(path: str, nprocs: int = 1, rank: int = 0, header_type: int = 1,
+                   h: Handle | None = None) -> CSR:
+    """Rank `rank`'s block of a binary COO file as a device CSR (int32 row
+    pointer, FP64), row_shift = the rows of the ranks before it."""
+    h = h or default_handle()
+    s, X = h.coo_read(path, nprocs, rank, header_type)
+    return CSR.from_device(X, full_n=int(s.nr), row_shift=int(s.col_shift))
 
 
 def rank_rows(nr: int, nprocs: int, rank: int) -> tuple[int, int]:

@@ -20,6 +20,7 @@
 #include "device_utils.hpp"
 #include "nsp_internal.hpp"
 #include "scan.hpp"
+#include "segsort.hpp"
 #include "spgemm_internal.hpp"
 
 namespace nsp {
@@ -55,20 +56,22 @@ __global__ void k_t_scatter(int64_t rows, const RowT *__restrict__ rpt, const in
 }
 
 // One block of 256 threads per X^T row: bitonic sort of (row index, value)
-// pairs in LDS, rows up to kTSortMax entries.
+// pairs in LDS, rows up to kTSortMax entries; longer rows are listed for the
+// merge sort of sort_listed_rows.
 constexpr int kTSortMax = 4096;
 
 template <typename RowT, typename V>
 __global__ __launch_bounds__(256) void k_t_sort(int64_t trows, const RowT *__restrict__ trpt,
                                                 int32_t *__restrict__ tcol, V *__restrict__ tval,
-                                                uint32_t *__restrict__ too_long) {
+                                                unsigned long long *__restrict__ nlong,
+                                                int64_t *__restrict__ long_rows) {
   __shared__ uint64_t S[kTSortMax];  // (row index << 32) | position in the row
   __shared__ V Vt[kTSortMax];
   for (int64_t r = blockIdx.x; r < trows; r += gridDim.x) {
     const int64_t p0 = (int64_t)trpt[r], n = (int64_t)trpt[r + 1] - p0;
     if (n <= 1) continue;
     if (n > kTSortMax) {
-      if (threadIdx.x == 0) *too_long = 1;
+      if (threadIdx.x == 0) long_rows[atomicAdd(nlong, 1ull)] = r;
       continue;
     }
     int n2 = 1;
@@ -123,29 +126,31 @@ static int transpose_impl(Handle *h, const nsp_csr_t *X, nsp_csr_t *XT) {
   const int64_t n = X->cols, nnz = X->nnz;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = n / kScanTile + 2;
-  NSP_TRY(aux2_reserve(h, al((n + 1) * 8) + al(ntiles * 8) + 256));
+  const int64_t long_cap = nnz / (kTSortMax + 1) + 1;  // rows of X^T past kTSortMax
+  NSP_TRY(aux2_reserve(h, al((n + 1) * 8) + al(ntiles * 8) + 256 + al(long_cap * 8)));
   char *p = (char *)h->aux2.base;
   int64_t *cnt = (int64_t *)p;
   uint64_t *status = (uint64_t *)(p + al((n + 1) * 8));
   uint32_t *ticket = (uint32_t *)(p + al((n + 1) * 8) + al(ntiles * 8));
-  uint32_t *too_long = ticket + 1;
+  unsigned long long *nlong = (unsigned long long *)(ticket + 2);
+  int64_t *long_rows = (int64_t *)(p + al((n + 1) * 8) + al(ntiles * 8) + 256);
   hipStream_t st = h->stream;
   const int g =(int)std::max<int64_t>(1, std::min<int64_t>((std::max(n, nnz) + 255) / 256, (int64_t)h->num_cu * 16));
   hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, status, ntiles, ticket);
   hipLaunchKernelGGL((k_t_count<RowT>), dim3(g), dim3(256), 0, st, nnz, X->col, cnt);
   hipLaunchKernelGGL((k_exscan<int64_t, RowT>), dim3((unsigned)(n / kScanTile + 1)), dim3(kScanBS), 0, st, n, cnt,
                      (RowT *)XT->row_ptr, status, ticket);
-  // reuse cnt as the scatter cursor
-  hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, status, 0, too_long);
+  // reuse cnt as the scatter cursor; nlong = 0
+  hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, (uint64_t *)nlong, 1, ticket + 1);
   hipLaunchKernelGGL((k_t_scatter<RowT, V>), dim3(g), dim3(256), 0, st, X->rows, (const RowT *)X->row_ptr, X->col,
                      (const V *)X->val, (const RowT *)XT->row_ptr, cnt, XT->col, (V *)XT->val);
   hipLaunchKernelGGL((k_t_sort<RowT, V>), dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)h->num_cu * 8))),
-                     dim3(256), 0, st, n, (const RowT *)XT->row_ptr, XT->col, (V *)XT->val, too_long);
+                     dim3(256), 0, st, n, (const RowT *)XT->row_ptr, XT->col, (V *)XT->val, nlong, long_rows);
   NSP_HIP(h, hipGetLastError());
-  uint32_t tl = 0;
-  NSP_HIP(h, hipMemcpyAsync(&tl, too_long, 4, hipMemcpyDeviceToHost, st));
+  unsigned long long nl = 0;
+  NSP_HIP(h, hipMemcpyAsync(&nl, nlong, 8, hipMemcpyDeviceToHost, st));
   NSP_HIP(h, hipStreamSynchronize(st));
-  if (tl) return set_error(h, NSP_ERR_UNSUPPORTED, "transpose: a column holds more than 4096 entries");
+  if (nl > 0) return sort_listed_rows<RowT, V>(h, long_rows, (int64_t)nl, (const RowT *)XT->row_ptr, XT->col, (V *)XT->val);
   return NSP_OK;
 }
 

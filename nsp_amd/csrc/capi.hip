@@ -9,10 +9,6 @@
 #include "nsp_internal.hpp"
 #include "spgemm_internal.hpp"
 
-struct nsp_handle_s {
-  nsp::Handle h;
-};
-
 namespace nsp {
 
 int set_error(Handle *h, int status, const std::string &msg) {
@@ -111,6 +107,12 @@ int nsp_destroy(nsp_handle_t hs) {
   if (h->aux.base) (void)hipFree(h->aux.base);
   if (h->aux2.base) (void)hipFree(h->aux2.base);
   if (h->bwd.base) (void)hipFree(h->bwd.base);
+  if (h->segw.base) (void)hipFree(h->segw.base);
+  if (h->coo.base) (void)hipFree(h->coo.base);
+  if (h->rsort.base) (void)hipFree(h->rsort.base);
+  if (h->pin) (void)hipHostFree(h->pin);
+  for (hipEvent_t e : h->pin_ev)
+    if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->events)
     if (e) (void)hipEventDestroy(e);
   if (h->d_info) (void)hipFree(h->d_info);
@@ -181,6 +183,10 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
   } else if (!std::strcmp(name, "heavy_blocks")) {
     if (value < 0 || value > 65536) return nsp::set_error(h, NSP_ERR_INVALID, "heavy_blocks must be in [0, 65536]");
     h->heavy_blocks = (int)value;
+  } else if (!std::strcmp(name, "coo_chunk_records")) {
+    if (value < 1 || value > (int64_t(1) << 26))
+      return nsp::set_error(h, NSP_ERR_INVALID, "coo_chunk_records must be in [1, 2^26]");
+    h->coo_chunk = value;
   } else if (!std::strcmp(name, "grid_cap")) {
     if (value < 1 || value > 64) return nsp::set_error(h, NSP_ERR_INVALID, "grid_cap must be in [1, 64]");
     h->grid_cap = (int)value;

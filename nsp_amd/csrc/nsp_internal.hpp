@@ -74,6 +74,7 @@ struct Handle {
   int64_t ext_rows = -1, ext_nnz = -1;
   int64_t ext_nprod = -1;  // nprod of the cached extents (known once read back)
   int num_table_factor = 2;
+  int64_t coo_chunk = int64_t(1) << 21;  // records per streamed chunk of the COO reader
   bool timing = false;
   bool timing_pending = false;
   hipEvent_t events[8] = {};
@@ -83,6 +84,12 @@ struct Handle {
   Workspace aux;
   Workspace aux2;  // transpose
   Workspace bwd;   // backward: flags, row lists, product offsets, contributions
+  Workspace segw;  // segment sort: tile lists, merge buffer
+  Workspace rsort; // row sort: extents, keys, values of the rows being sorted
+  Workspace coo;   // COO conversion: row bitmap, unsorted-row list, staging
+  void *pin = nullptr;  // pinned host staging of the COO reader
+  size_t pin_bytes = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
   bool poff_valid = false;  // bwd holds the product offsets of the cached extents
   const void *bwd_c_key[2] = {nullptr, nullptr};  // C pattern whose rows passed the length check
   Info *d_info = nullptr;
@@ -103,6 +110,8 @@ struct Handle {
   int64_t ntiles = 0;
 };
 
+inline Handle *handle_of(nsp_handle_t hs);
+
 int set_error(Handle *h, int status, const std::string &msg);
 int collect_timing(Handle *h);
 
@@ -121,3 +130,10 @@ int collect_timing(Handle *h);
   } while (0)
 
 }  // namespace nsp
+
+// The opaque handle of the C ABI.
+struct nsp_handle_s {
+  nsp::Handle h;
+};
+
+inline nsp::Handle *nsp::handle_of(nsp_handle_t hs) { return &hs->h; }

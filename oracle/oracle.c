@@ -376,9 +376,14 @@ void orc_backward(int64_t M, const int64_t *arpt, const int32_t *acol, const dou
  * records {int64 i, int64 j, double v} with 1-based i, j, sorted by row.
  * Rank `rank` of `nprocs` owns rows [nr/nprocs*rank, ...), the last rank the
  * remainder (READ_MATRIX_MPI.cpp:256-270); its records are located by binary
- * search on the row id (binsearch_iat, 83-130).  Columns become 0-based and
- * are shifted by the rank's first present row (distribute.cu:1096-1105); rows
- * with unsorted columns are sorted (check_and_fix_order, 1050-1069).
+ * search on the row id (binsearch_iat, 83-130).  The rank's rows run from
+ * its first to its last present row (rowxproc, distribute.cu:1094); columns
+ * become 0-based and are shifted by col_shift = the sum of rowxproc over the
+ * ranks before it (the MPI_Exscan at distribute.cu:1095, applied at 1103 and
+ * stored as the CSR's row_shift at 1105), which equals the rank's first row
+ * unless empty rows sit at partition ends.  A rank with no records spans no
+ * rows (the reference reads irow[-1] there).  Rows with unsorted columns are
+ * sorted (check_and_fix_order, 1050-1069).
  * Returns 0 on success; on success *out_* are malloc'd and owned by the
  * caller (free with orc_free). */
 typedef struct { int64_t i, j; double v; } orc_coo_rec;
@@ -412,6 +417,21 @@ int orc_read_binary_coo(const char *path, int header_type, int nprocs, int rank,
     int64_t b = coo_lower_bound(f, hdr, nt, first + 1);
     int64_t e = coo_lower_bound(f, hdr, nt, last + 1);
     if (b < 0 || e < 0) { fclose(f); return 3; }
+    /* col_shift: rows spanned by the ranks before this one */
+    int64_t col_shift = 0;
+    for (int r = 0; r < rank; r++) {
+        int64_t rb = coo_lower_bound(f, hdr, nt, per * r + 1);
+        int64_t re = coo_lower_bound(f, hdr, nt, per * (r + 1) + 1);
+        if (rb < 0 || re < 0) { fclose(f); return 3; }
+        if (re > rb) {
+            orc_coo_rec x, y;
+            fseeko(f, (off_t)(hdr + rb * (int64_t)sizeof(orc_coo_rec)), SEEK_SET);
+            if (fread(&x, sizeof x, 1, f) != 1) { fclose(f); return 3; }
+            fseeko(f, (off_t)(hdr + (re - 1) * (int64_t)sizeof(orc_coo_rec)), SEEK_SET);
+            if (fread(&y, sizeof y, 1, f) != 1) { fclose(f); return 3; }
+            col_shift += y.i - x.i + 1;
+        }
+    }
     int64_t n = e - b;
     orc_coo_rec *rec = (orc_coo_rec *)malloc(sizeof(orc_coo_rec) * (size_t)(n > 0 ? n : 1));
     fseeko(f, (off_t)(hdr + b * (int64_t)sizeof(orc_coo_rec)), SEEK_SET);
@@ -427,7 +447,7 @@ int orc_read_binary_coo(const char *path, int header_type, int nprocs, int rank,
     for (int64_t q = 0; q < n; q++) rpt[rec[q].i - 1 - r_first + 1]++;
     for (int64_t r = 0; r < nloc; r++) rpt[r + 1] += rpt[r];
     for (int64_t q = 0; q < n; q++) {
-        col[q] = (int32_t)(rec[q].j - 1 - r_first);
+        col[q] = (int32_t)(rec[q].j - 1 - col_shift);
         val[q] = rec[q].v;
     }
     free(rec);
@@ -441,7 +461,7 @@ int orc_read_binary_coo(const char *path, int header_type, int nprocs, int rank,
         }
     }
     *out_nr = nr; *out_nc = nc; *out_nt = nt;
-    *out_nloc = nloc; *out_row_shift = r_first; *out_nnz = n;
+    *out_nloc = nloc; *out_row_shift = col_shift; *out_nnz = n;
     *out_rpt = rpt; *out_col = col; *out_val = val;
     return 0;
 }

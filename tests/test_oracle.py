@@ -323,6 +323,25 @@ def test_reader_drops_empty_rows_at_block_ends(orc, tmp_path):
     assert shift == 0 and X.nrows == 2
 
 
+def test_reader_shift_is_exscan_of_spanned_rows(orc, tmp_path):
+    """Reference rule (src/distribute.cu:1094-1105): a rank's columns and
+    row_shift are the sum of rowxproc (first to last present row) over the
+    ranks before it.  Row 3 of 8 is empty, so with 2 ranks rank 0 spans 3
+    rows and rank 1's shift is 3, not its first row 4."""
+    p = str(tmp_path / "x.bin")
+    recs = np.array([(i + 1, i + 1, float(i)) for i in (0, 1, 2, 4, 5, 6, 7)],
+                    dtype=[("i", "<i8"), ("j", "<i8"), ("v", "<f8")])
+    with open(p, "wb") as f:
+        f.write(np.array([8, 8, len(recs)], "<i8").tobytes())
+        f.write(recs.tobytes())
+    _, _, _, s0, X0 = orc.read_binary_coo(p, nprocs=2, rank=0)
+    _, _, _, s1, X1 = orc.read_binary_coo(p, nprocs=2, rank=1)
+    assert (s0, X0.nrows, list(X0.rpt)) == (0, 3, [0, 1, 2, 3])
+    assert (s1, X1.nrows, list(X1.rpt)) == (3, 4, [0, 1, 2, 3, 4])
+    assert list(X1.col) == [4 - 3, 5 - 3, 6 - 3, 7 - 3]   # global col (0-based) minus 3
+    assert list(X1.val) == [4.0, 5.0, 6.0, 7.0]
+
+
 def test_reader_sorts_unsorted_rows(orc, tmp_path):
     p = str(tmp_path / "u.bin")
     recs = np.array([(1, 3, 1.0), (1, 1, 2.0), (2, 2, 3.0)],
