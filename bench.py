@@ -290,6 +290,32 @@ def rmat_leg(scale: int = 20, steps: int = 3, warmup: int = 1):
     return out
 
 
+def galerkin_leg(m: int = 64, steps: int = 10, warmup: int = 2):
+    """Coarse operator P^T (A P) of the 27-point stencil m^3 with the
+    aggregation prolongator (n x n/8), FP64: the R(AP) product the
+    reference's driver names as its purpose.  One step = nsp_galerkin (A P,
+    the transpose of P, P^T (AP), callee-allocated results)."""
+    import torch
+    import nsp_amd
+    from nsp_amd import gen
+    A = gen.stencil_27pt(m, seed=0)
+    P = gen.aggregation(A.nrows, seed=1)
+    dA = nsp_amd.DeviceCSR.from_host(A, "f64", "i32")
+    dP = nsp_amd.DeviceCSR.from_host(P, "f64", "i32")
+    h = nsp_amd.Handle()
+    C = h.galerkin(dA, dP)
+    _, n1 = h.intprod(dA, dP)
+    AP = h.spgemm(dA, dP)
+    PT = h.transpose(dP)
+    _, n2 = h.intprod(PT, AP)
+    del AP, PT
+    ms = _timed(lambda: h.galerkin(dA, dP), steps, warmup)
+    return {"workload": f"P^T (A P): 27-point stencil {m}^3 ({A.nrows} rows, {A.nnz} entries), aggregation P "
+                        f"({P.nrows}x{P.ncols}), FP64",
+            "nnz_C": C.nnz, "rows_C": C.nrows, "nprod": n1 + n2, "ms_per_product": round(ms, 4),
+            "GFLOPs": round(2.0 * (n1 + n2) / ms / 1e6, 1)}
+
+
 def coo_leg(A, B, steps: int = 10, reps: int = 3):
     """The reference's binary COO loader on the device: the file holds the
     stencil's rows [0, n/4 + m^2) (header: the whole matrix's n rows) and rank
@@ -359,8 +385,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="fwd_bwd,batched,rmat,coo",
-                    help="extra measurements after the headline: fwd_bwd, batched, rmat, coo "
+    ap.add_argument("--legs", default="fwd_bwd,batched,rmat,coo,galerkin",
+                    help="extra measurements after the headline: fwd_bwd, batched, rmat, coo, galerkin "
                          "(comma list, or none)")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's pick")
@@ -469,6 +495,8 @@ def main():
     if "coo" in legs:
         torch.cuda.empty_cache()
         out["coo"] = coo_leg(A, B)
+    if "galerkin" in legs:
+        out["galerkin"] = galerkin_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, B, nprod)
     print(json.dumps(out), flush=True)

@@ -216,6 +216,16 @@ int nsp_spgemm_backward(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const ns
                         const nsp_csr_t *B, const nsp_csr_t *AT, const void *at_perm,
                         const nsp_csr_t *C, const void *dC, void *dA, void *dB);
 
+/* ---- chained products ----------------------------------------------------- */
+/* Galerkin (coarse-operator) product C = R * (A * P), with R = P^T built on
+ * the device (nsp_transpose) when R is NULL; callee-allocated C as in
+ * nsp_spgemm (the intermediate A*P and P^T are freed before returning).
+ * No reference counterpart: the reference's driver computes A*P only and
+ * names R(AP) as the product it serves (main.cu:138); src/setting.h:63
+ * (GALERKIN_PRODUCT_TYPE) is the switch of the AMG code it comes from. */
+int nsp_galerkin(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *R, const nsp_csr_t *A,
+                 const nsp_csr_t *P, nsp_csr_t *C, nsp_alloc_fn alloc, nsp_free_fn dealloc, void *user);
+
 /* ---- one rank's local share on one device ------------------------------- */
 /* Column compaction ("shrink"): perm = sorted unique(col U [0..l]),
  * new_col[e] = position of col[e] in perm, *shrinked_m = |perm|.  Every

@@ -16,6 +16,9 @@
  *                                     src/distribute.cu:1071-1125 (host CSR)
  *   load_MatrixMPI                    src/distribute.cu:1071-1125 (device CSR)
  *   write_binary_matrix               main.cu:468-512
+ *   CSRMatrixPrintMM                  src/matrixIO.cu:299-339
+ *   append_outtime                    main.cu:236-247 (the outtime.txt line)
+ *   galerkin_product                  R(AP), named at main.cu:138
  * Errors: where the reference exits (checkCudaErrors, helper_cuda.h:714) these
  * functions throw nspamd::Error carrying the C ABI status.  Calls are
  * host-synchronous on return, as in the reference; each host thread uses its
@@ -139,6 +142,12 @@ CSR *nsparseMGPU_version0(CSR *A, CSR *P);
 CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local_n = -1, CSR **A_out = nullptr,
                           CSR **P_out = nullptr);
 
+/* Galerkin product C = R * (A * P) on the device, R = P^T when R is NULL
+ * (nsp_galerkin): the coarse operator the reference's driver names as its
+ * purpose (main.cu:138, "il prodotto e' R(AP)").  C is a device CSR
+ * (CSRm::free). */
+CSR *galerkin_product(CSR *R, CSR *A, CSR *P);
+
 /* Rank `rank` of `nprocs` of a binary COO matrix file (header {nr, nc, nt}
  * when header_type = 1, {nr, nt} when 0, then nt records {int64 i, int64 j,
  * double v}, 1-based, sorted by row): host CSR whose n rows run from the
@@ -154,8 +163,16 @@ CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type 
  * binary files, row partition). */
 CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type = 1);
 /* Writes a host CSR as binary COO (header_type 1); global column =
- * col + col_shift. */
+ * col + col_shift.  The layout the reader takes: the reference's
+ * write_binary_matrix template (main.cu:468-512) with otype = long. */
 void write_binary_matrix(const char *path, const CSR *A, int64_t col_shift = 0);
+/* MatrixMarket coordinate file of A (host or device) with the reference's
+ * header and line formats (src/matrixIO.cu:299-339). */
+void CSRMatrixPrintMM(CSR *A, const char *file_name);
+/* Appends the reference's timing line (main.cu:243-247: rows, columns and
+ * nnz of the product, local and total time, ||C||_F) to path. */
+void append_outtime(const char *path, uint64_t tot_n, int64_t m, uint64_t tot_nnz, double loc_time, double tot_time,
+                    double l2n);
 
 }  // namespace nspamd
 

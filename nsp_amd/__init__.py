@@ -243,6 +243,35 @@ class Handle:
                                                 int(row_offset), int(col_shift), C.byref(o)))
         return out
 
+    # -- chained products ---------------------------------------------------
+    def galerkin(self, A: DeviceCSR, P: DeviceCSR, R: DeviceCSR | None = None) -> DeviceCSR:
+        """C = R (A P), R = P^T (built on the device) when R is None
+        (nsp_galerkin); C's arrays live in torch's allocator."""
+        torch = _torch()
+        dev = A.col.device
+        keep = {}
+
+        def _alloc(nbytes, user):
+            t = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+            keep[t.data_ptr()] = t
+            return t.data_ptr()
+
+        def _free(ptr, user):
+            keep.pop(ptr, None)
+
+        alloc, free = _lib.ALLOC_FN(_alloc), _lib.FREE_FN(_free)
+        a, p = A.c_struct(), P.c_struct()
+        r = R.c_struct() if R is not None else None
+        c = _lib.nsp_csr_t()
+        self._check(self.L.nsp_galerkin(self.h, A.index_type, A.value_type, C.byref(r) if r is not None else None,
+                                        C.byref(a), C.byref(p), C.byref(c), C.cast(alloc, C.c_void_p),
+                                        C.cast(free, C.c_void_p), None))
+        nnz, rows = int(c.nnz), int(c.rows)
+        rpt = keep[c.row_ptr].view(A.row_ptr.dtype)[:rows + 1]
+        col = keep[c.col].view(torch.int32)[:nnz] if c.col else torch.empty(0, dtype=torch.int32, device=dev)
+        val = keep[c.val].view(A.val.dtype)[:nnz] if c.val else torch.empty(0, dtype=A.val.dtype, device=dev)
+        return DeviceCSR(rpt, col, val, rows, int(c.cols))
+
     # -- binary COO files ---------------------------------------------------
     def coo_to_csr(self, records, first_row: int, nrows: int, col_shift: int = 0, ncols: int = 0,
                    index: str = "i32", dtype: str = "f64") -> DeviceCSR:

@@ -74,10 +74,16 @@ SIGNATURES = [
     ("nsp_graph_destroy", _I, [_H]),
     ("nsp_transpose", _I, [_H, _I, _I, _CSRP, _CSRP]),
     ("nsp_spgemm_backward", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _VP, _CSRP, _VP, _VP, _VP]),
+    ("nsp_galerkin", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _CSRP, _VP, _VP, _VP]),
     ("nsp_coo_share", _I, [C.c_char_p, _I, _I, _I, C.POINTER(nsp_coo_share_t)]),
     ("nsp_coo_to_csr", _I, [_H, _I, _I, _VP, _I64, _I64, _I64, _CSRP]),
     ("nsp_coo_read", _I, [_H, _I, _I, C.c_char_p, _I, C.POINTER(nsp_coo_share_t), _CSRP]),
 ]
+
+
+# allocation callbacks of the callee-allocating entry points
+ALLOC_FN = C.CFUNCTYPE(C.c_void_p, C.c_size_t, C.c_void_p)
+FREE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
 
 
 def load():

@@ -285,6 +285,27 @@ CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local
 }
 
 // ---------------------------------------------------------------------------
+CSR *galerkin_product(CSR *R, CSR *A, CSR *P) {
+  if (!A || !P || !A->on_the_device || !P->on_the_device || (R && !R->on_the_device))
+    throw Error(NSP_ERR_INVALID, "galerkin_product: device matrices expected");
+  nsp_handle_t h = handle();
+  const nsp_csr_t a = view(A, false), p = view(P, false);
+  nsp_csr_t r{}, c{};
+  if (R) r = view(R, false);
+  check(nsp_galerkin(h, NSP_IDX32, NSP_F64, R ? &r : nullptr, &a, &p, &c, nullptr, nullptr, nullptr), h,
+        "galerkin_product");
+  if (c.nnz > INT32_MAX) {
+    nsp_csr_free(&c);
+    throw Error(NSP_ERR_OVERFLOW, "galerkin_product: nnz does not fit CSR's int");
+  }
+  CSR *C = CSRm::init((int)c.rows, c.cols, (int)c.nnz, false, true, c.rows, 0);
+  C->row = (int *)c.row_ptr;
+  C->col = c.col;
+  C->val = (double *)c.val;
+  return C;
+}
+
+// ---------------------------------------------------------------------------
 namespace {
 
 struct CooRec {
@@ -367,6 +388,34 @@ CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type) {
     check(st, h, "load_MatrixMPI");
   }
   return A;
+}
+
+void CSRMatrixPrintMM(CSR *A_, const char *file_name) {
+  CSR *A = A_->on_the_device ? CSRm::copyToHost(A_) : A_;
+  FILE *fp = fopen(file_name, "w");
+  if (!fp) {
+    if (A != A_) CSRm::free(A);
+    throw Error(NSP_ERR_INVALID, std::string("CSRMatrixPrintMM: cannot open ") + file_name);
+  }
+  // the reference's formats (matrixIO.cu:328-336), 1-based indices
+  fprintf(fp, "%s\n", "%%MatrixMarket matrix coordinate real general");
+  fprintf(fp, "%d  %lld %d \n", A->n, (long long)A->m, A->nnz);
+  for (int i = 0; i < A->n; i++)
+    for (int j = A->row[i]; j < A->row[i + 1]; j++) fprintf(fp, "%d   %d  %lg\n", i + 1, A->col[j] + 1, A->val[j]);
+  const bool ok = !ferror(fp);
+  fclose(fp);
+  if (A != A_) CSRm::free(A);
+  if (!ok) throw Error(NSP_ERR_INVALID, std::string("CSRMatrixPrintMM: write failed on ") + file_name);
+}
+
+void append_outtime(const char *path, uint64_t tot_n, int64_t m, uint64_t tot_nnz, double loc_time, double tot_time,
+                    double l2n) {
+  FILE *fid = fopen(path, "a");
+  if (!fid) throw Error(NSP_ERR_INVALID, std::string("append_outtime: cannot open ") + path);
+  // main.cu:243-247
+  fprintf(fid, "%15lu %15lu %15lu ", (unsigned long)tot_n, (unsigned long)m, (unsigned long)tot_nnz);
+  fprintf(fid, "%10f %10f %50f\n", loc_time, tot_time, l2n);
+  fclose(fid);
 }
 
 void write_binary_matrix(const char *path, const CSR *A, int64_t col_shift) {

@@ -4,15 +4,20 @@
 //
 //   nsp_amd_run product A.bin B.bin [C.bin]
 //       C = A*B through nsp_spgemm_kernel_hash (sfCSR interface)
-//   nsp_amd_run local A.bin P.bin nprocs rank [C.bin]
+//   nsp_amd_run local A.bin P.bin nprocs rank [C.bin|-] [outtime.txt]
 //       rank `rank` of `nprocs`: its row block of A times the P rows it needs
-//       (nsparseMGPU_version1 without MPI: P is read whole onto the device)
+//       (nsparseMGPU_version1 without MPI: P is read whole onto the device);
+//       prints main.cu's TOTAL_TIME line and appends its outtime.txt line
 //   nsp_amd_run split A.bin nprocs rank out.bin
 //       the reader/splitter alone (host only): the rank's block, columns in
 //       the rank frame, written back as binary COO
 //   nsp_amd_run load A.bin nprocs rank out.bin
 //       the same block built on the device (load_MatrixMPI), copied back
 //       and written
+//   nsp_amd_run galerkin A.bin P.bin [C.bin]
+//       the coarse operator P^T (A P) (galerkin_product)
+//   nsp_amd_run mm A.bin out.mtx
+//       A as a MatrixMarket file (CSRMatrixPrintMM; host only)
 //
 // Matrices are binary COO files (header {nr, nc, nt}, then {i, j, v}
 // records, 1-based, sorted by row).  Prints one JSON line; C.bin, when given,
@@ -92,7 +97,11 @@ static int run_local(int argc, char **argv) {
   CSR *A_ = nullptr, *P_ = nullptr;
   CSR *C = nsparseMGPU_version1(Ad, Pd, loc_time, (int)pshare.nrows, &A_, &P_);
   const double l2 = CSRm::norm_fro(C);
-  if (argc > 6) write_device_csr(argv[6], C);
+  if (argc > 6 && strcmp(argv[6], "-") != 0) write_device_csr(argv[6], C);
+  // main.cu:241-247: the summary line and, when asked, the outtime.txt line
+  // (one rank: loc_time is also the total)
+  printf("TOTAL_TIME: %f\t\tlocal time: %f\t\tnnz = %lu\t\tl2norm: %f\n", loc_time, loc_time, (unsigned long)C->nnz, l2);
+  if (argc > 7) append_outtime(argv[7], (uint64_t)C->n, C->m, (uint64_t)C->nnz, loc_time, loc_time, l2);
   printf("{\"mode\": \"local\", \"nprocs\": %d, \"rank\": %d, \"row_shift\": %lld, \"rows\": %d, "
          "\"shrinked_m_A\": %d, \"completedP_rows\": %d, \"shrinked_m_P\": %d, \"nnz\": %d, "
          "\"l2norm\": %.17g, \"loc_time\": %.6f, \"read_s\": %.6f}\n",
@@ -103,6 +112,33 @@ static int run_local(int argc, char **argv) {
   CSRm::freeStruct(A_);
   CSRm::free(Ad);
   CSRm::free(Pd);
+  return 0;
+}
+
+static int run_mm(int argc, char **argv) {
+  if (argc < 4) return 2;
+  CSR *A = read_binary_matrix(argv[2], 1, 0);
+  CSRMatrixPrintMM(A, argv[3]);
+  printf("{\"mode\": \"mm\", \"rows\": %d, \"nnz\": %d}\n", A->n, A->nnz);
+  CSRm::free(A);
+  return 0;
+}
+
+static int run_galerkin(int argc, char **argv) {
+  if (argc < 4) return 2;
+  CSR *A = load_MatrixMPI(argv[2], 1, 0);
+  CSR *P = load_MatrixMPI(argv[3], 1, 0);
+  const auto t0 = std::chrono::steady_clock::now();
+  CSR *C = galerkin_product(nullptr, A, P);
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const double l2 = CSRm::norm_fro(C);
+  if (argc > 4) write_device_csr(argv[4], C);
+  printf("{\"mode\": \"galerkin\", \"rows\": %d, \"cols\": %lld, \"nnz\": %d, \"l2norm\": %.17g, "
+         "\"time_s\": %.6f}\n",
+         C->n, (long long)C->m, C->nnz, l2, t);
+  CSRm::free(C);
+  CSRm::free(A);
+  CSRm::free(P);
   return 0;
 }
 
@@ -143,7 +179,7 @@ int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr,
             "usage: %s product A.bin B.bin [C.bin] | local A.bin P.bin nprocs rank [C.bin] | "
-            "split|load A.bin nprocs rank out.bin\n",
+            "split|load A.bin nprocs rank out.bin | galerkin A.bin P.bin [C.bin] | mm A.bin out.mtx\n",
             argv[0]);
     return 2;
   }
@@ -153,6 +189,8 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[1], "local")) rc = run_local(argc, argv);
     else if (!strcmp(argv[1], "split")) rc = run_split(argc, argv, false);
     else if (!strcmp(argv[1], "load")) rc = run_split(argc, argv, true);
+    else if (!strcmp(argv[1], "galerkin")) rc = run_galerkin(argc, argv);
+    else if (!strcmp(argv[1], "mm")) rc = run_mm(argc, argv);
     if (rc == 2) fprintf(stderr, "bad arguments\n");
     return rc;
   } catch (const Error &e) {

@@ -55,25 +55,70 @@ __global__ void k_t_scatter(int64_t rows, const RowT *__restrict__ rpt, const in
   }
 }
 
-// One block of 256 threads per X^T row: bitonic sort of (row index, value)
-// pairs in LDS, rows up to kTSortMax entries; longer rows are listed for the
-// merge sort of sort_listed_rows.
+// Rows of X^T are sorted by their (original row) index in three tiers:
+// up to 64 entries by one wavefront in registers (k_t_sort_small), up to
+// kTSortMax by one block in LDS (k_t_sort, rows listed by the first tier),
+// longer ones by the merge sort of sort_listed_rows (listed as well).
 constexpr int kTSortMax = 4096;
 
+// One wavefront per row of up to 64 entries: a bitonic network over the 64
+// lanes on keys (row index << 32 | position); values follow through a
+// shuffle from the lane that held them.  Rows past 64 entries are listed.
 template <typename RowT, typename V>
-__global__ __launch_bounds__(256) void k_t_sort(int64_t trows, const RowT *__restrict__ trpt,
-                                                int32_t *__restrict__ tcol, V *__restrict__ tval,
-                                                unsigned long long *__restrict__ nlong,
-                                                int64_t *__restrict__ long_rows) {
-  __shared__ uint64_t S[kTSortMax];  // (row index << 32) | position in the row
-  __shared__ V Vt[kTSortMax];
-  for (int64_t r = blockIdx.x; r < trows; r += gridDim.x) {
+__global__ __launch_bounds__(256) void k_t_sort_small(int64_t trows, const RowT *__restrict__ trpt,
+                                                      int32_t *__restrict__ tcol, V *__restrict__ tval,
+                                                      unsigned long long *__restrict__ counts,
+                                                      int64_t *__restrict__ mid_rows,
+                                                      int64_t *__restrict__ long_rows) {
+  const int lane = lane_id();
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t r = wave; r < trows; r += nwaves) {
     const int64_t p0 = (int64_t)trpt[r], n = (int64_t)trpt[r + 1] - p0;
     if (n <= 1) continue;
-    if (n > kTSortMax) {
-      if (threadIdx.x == 0) long_rows[atomicAdd(nlong, 1ull)] = r;
+    if (n > 64) {
+      if (lane == 0) {
+        if (n > kTSortMax) long_rows[atomicAdd(&counts[1], 1ull)] = r;
+        else mid_rows[atomicAdd(&counts[0], 1ull)] = r;
+      }
       continue;
     }
+    uint64_t key = ~0ull;
+    V v = (V)0;
+    if (lane < n) {
+      key = ((uint64_t)(uint32_t)tcol[p0 + lane] << 32) | (uint32_t)lane;
+      v = tval[p0 + lane];
+    }
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const uint64_t o = __shfl_xor(key, j);
+        const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+        key = (lower == up) ? (key < o ? key : o) : (key < o ? o : key);
+      }
+    }
+    const V w = __shfl(v, (int)((uint32_t)key & 63));
+    if (lane < n) {
+      tcol[p0 + lane] = (int32_t)(key >> 32);
+      tval[p0 + lane] = w;
+    }
+  }
+}
+
+// One block of 256 threads per listed row of 65..kTSortMax entries: bitonic
+// sort of (row index, position) keys in LDS, values gathered by position.
+template <typename RowT, typename V>
+__global__ __launch_bounds__(256) void k_t_sort(const int64_t *__restrict__ rows,
+                                                const unsigned long long *__restrict__ nrows,
+                                                const RowT *__restrict__ trpt, int32_t *__restrict__ tcol,
+                                                V *__restrict__ tval) {
+  __shared__ uint64_t S[kTSortMax];  // (row index << 32) | position in the row
+  __shared__ V Vt[kTSortMax];
+  const int64_t count = (int64_t)*nrows;
+  for (int64_t qi = blockIdx.x; qi < count; qi += gridDim.x) {
+    const int64_t r = rows[qi];
+    const int64_t p0 = (int64_t)trpt[r], n = (int64_t)trpt[r + 1] - p0;
     int n2 = 1;
     while (n2 < n) n2 <<= 1;
     for (int g = threadIdx.x; g < n2; g += 256) {
@@ -127,28 +172,34 @@ static int transpose_impl(Handle *h, const nsp_csr_t *X, nsp_csr_t *XT) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = n / kScanTile + 2;
   const int64_t long_cap = nnz / (kTSortMax + 1) + 1;  // rows of X^T past kTSortMax
-  NSP_TRY(aux2_reserve(h, al((n + 1) * 8) + al(ntiles * 8) + 256 + al(long_cap * 8)));
+  const int64_t mid_cap = nnz / 65 + 1;                 // rows of X^T past 64 entries
+  NSP_TRY(aux2_reserve(h, al((n + 1) * 8) + al(ntiles * 8) + 256 + al(long_cap * 8) + al(mid_cap * 8)));
   char *p = (char *)h->aux2.base;
   int64_t *cnt = (int64_t *)p;
   uint64_t *status = (uint64_t *)(p + al((n + 1) * 8));
   uint32_t *ticket = (uint32_t *)(p + al((n + 1) * 8) + al(ntiles * 8));
-  unsigned long long *nlong = (unsigned long long *)(ticket + 2);
+  unsigned long long *counts = (unsigned long long *)(ticket + 2);  // [0] mid rows, [1] long rows
   int64_t *long_rows = (int64_t *)(p + al((n + 1) * 8) + al(ntiles * 8) + 256);
+  int64_t *mid_rows = (int64_t *)(p + al((n + 1) * 8) + al(ntiles * 8) + 256 + al(long_cap * 8));
   hipStream_t st = h->stream;
   const int g =(int)std::max<int64_t>(1, std::min<int64_t>((std::max(n, nnz) + 255) / 256, (int64_t)h->num_cu * 16));
   hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, status, ntiles, ticket);
   hipLaunchKernelGGL((k_t_count<RowT>), dim3(g), dim3(256), 0, st, nnz, X->col, cnt);
   hipLaunchKernelGGL((k_exscan<int64_t, RowT>), dim3((unsigned)(n / kScanTile + 1)), dim3(kScanBS), 0, st, n, cnt,
                      (RowT *)XT->row_ptr, status, ticket);
-  // reuse cnt as the scatter cursor; nlong = 0
-  hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, (uint64_t *)nlong, 1, ticket + 1);
+  // reuse cnt as the scatter cursor; both row-list counts = 0
+  hipLaunchKernelGGL(k_t_zero, dim3(g), dim3(256), 0, st, n + 1, cnt, (uint64_t *)counts, 2, ticket + 1);
   hipLaunchKernelGGL((k_t_scatter<RowT, V>), dim3(g), dim3(256), 0, st, X->rows, (const RowT *)X->row_ptr, X->col,
                      (const V *)X->val, (const RowT *)XT->row_ptr, cnt, XT->col, (V *)XT->val);
-  hipLaunchKernelGGL((k_t_sort<RowT, V>), dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)h->num_cu * 8))),
-                     dim3(256), 0, st, n, (const RowT *)XT->row_ptr, XT->col, (V *)XT->val, nlong, long_rows);
+  const int gw = (int)std::max<int64_t>(1, std::min<int64_t>((n + 3) / 4, (int64_t)h->num_cu * 16));
+  hipLaunchKernelGGL((k_t_sort_small<RowT, V>), dim3(gw), dim3(256), 0, st, n, (const RowT *)XT->row_ptr, XT->col,
+                     (V *)XT->val, counts, mid_rows, long_rows);
+  hipLaunchKernelGGL((k_t_sort<RowT, V>), dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(mid_cap, (int64_t)h->num_cu * 8))),
+                     dim3(256), 0, st, (const int64_t *)mid_rows, (const unsigned long long *)counts,
+                     (const RowT *)XT->row_ptr, XT->col, (V *)XT->val);
   NSP_HIP(h, hipGetLastError());
   unsigned long long nl = 0;
-  NSP_HIP(h, hipMemcpyAsync(&nl, nlong, 8, hipMemcpyDeviceToHost, st));
+  NSP_HIP(h, hipMemcpyAsync(&nl, counts + 1, 8, hipMemcpyDeviceToHost, st));
   NSP_HIP(h, hipStreamSynchronize(st));
   if (nl > 0) return sort_listed_rows<RowT, V>(h, long_rows, (int64_t)nl, (const RowT *)XT->row_ptr, XT->col, (V *)XT->val);
   return NSP_OK;

@@ -1,6 +1,7 @@
 // extern "C" entry points of libnsp_amd.so (declared in include/nsp_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -260,6 +261,40 @@ int nsp_spgemm(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t 
     dealloc(val, user);
     std::memset(C, 0, sizeof *C);
   }
+  return s;
+}
+
+int nsp_galerkin(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *R, const nsp_csr_t *A,
+                 const nsp_csr_t *P, nsp_csr_t *C, nsp_alloc_fn alloc, nsp_free_fn dealloc, void *user) {
+  if (!hs || !C) return NSP_ERR_INVALID;
+  Handle *h = &hs->h;
+  if (!A || !P) return nsp::set_error(h, NSP_ERR_INVALID, "nsp_galerkin: A or P is NULL");
+  if (A->cols != P->rows) return nsp::set_error(h, NSP_ERR_INVALID, "nsp_galerkin: A.cols != P.rows");
+  if (R ? R->cols != A->rows : P->rows != A->rows)
+    return nsp::set_error(h, NSP_ERR_INVALID, "nsp_galerkin: R.cols (P.rows for R = P^T) != A.rows");
+  std::memset(C, 0, sizeof *C);
+  nsp_csr_t AP{};
+  int s = nsp_spgemm(hs, it, vt, A, P, &AP, nullptr, nullptr, nullptr);
+  if (s != NSP_OK) return s;
+  nsp_csr_t PT{};
+  if (!R) {
+    const size_t isz = (it == NSP_IDX64) ? 8 : 4, vsz = (vt == NSP_F32) ? 4 : 8;
+    PT.rows = P->cols;
+    PT.cols = P->rows;
+    PT.nnz = P->nnz;
+    if (hipMalloc(&PT.row_ptr, (size_t)(PT.rows + 1) * isz) != hipSuccess ||
+        hipMalloc((void **)&PT.col, std::max<size_t>((size_t)PT.nnz * 4, 4)) != hipSuccess ||
+        hipMalloc(&PT.val, std::max<size_t>((size_t)PT.nnz * vsz, 8)) != hipSuccess) {
+      (void)hipGetLastError();
+      s = nsp::set_error(h, NSP_ERR_OOM, "nsp_galerkin: P^T allocation failed");
+    } else {
+      s = nsp_transpose(hs, it, vt, P, &PT);
+    }
+    R = &PT;
+  }
+  if (s == NSP_OK) s = nsp_spgemm(hs, it, vt, R, &AP, C, alloc, dealloc, user);
+  (void)nsp_csr_free(&AP);
+  if (PT.row_ptr || PT.col || PT.val) (void)nsp_csr_free(&PT);
   return s;
 }
 

@@ -55,3 +55,21 @@ def test_split_matches_oracle_reader(orc, tmp_path, case):
             np.testing.assert_array_equal(rec["i"], rows)
             np.testing.assert_array_equal(rec["j"], ref.col.astype(np.int64) + 1)
             np.testing.assert_array_equal(rec["v"], ref.val)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="nsp_amd_run not built")
+def test_matrix_market_writer_formats(orc, tmp_path):
+    """CSRMatrixPrintMM writes the reference's header and line formats
+    (src/matrixIO.cu:328-336: "%d  %d %d \\n", "%d   %d  %lg\\n", 1-based)."""
+    X = gen.random_csr(40, 55, 0.08, seed=12, empty_rows=0.2)
+    X.val[::3] *= 1e-7          # exponents in %g
+    X.val[1::5] = np.round(X.val[1::5] * 100)   # integral values
+    path, out = str(tmp_path / "a.bin"), str(tmp_path / "a.mtx")
+    orc.write_binary_coo(path, X)
+    r = subprocess.run([DRIVER, "mm", path, out], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    lines = ["%%MatrixMarket matrix coordinate real general\n", "%d  %d %d \n" % (X.nrows, X.ncols, X.nnz)]
+    for i in range(X.nrows):
+        for p in range(X.rpt[i], X.rpt[i + 1]):
+            lines.append("%d   %d  %g\n" % (i + 1, X.col[p] + 1, X.val[p]))
+    assert open(out).read() == "".join(lines)

@@ -101,3 +101,43 @@ def test_load_on_device_equals_host_reader(orc, tmp_path, case):
             for k in ("nr", "nc", "nt", "rows", "nnz", "row_shift"):
                 assert mh[k] == md[k], k
             assert open(host_out, "rb").read() == open(dev_out, "rb").read()
+
+
+def test_galerkin_driver(orc, tmp_path):
+    """galerkin_product through the C++ API: P^T (A P) for the stencil and
+    the aggregation prolongator, written back and compared with two oracle
+    products."""
+    from nsp_amd.gen import HostCSR
+    A = gen.stencil_27pt(10, seed=6)
+    P = gen.aggregation(A.nrows, seed=3)
+    pa, pp, pc = (str(tmp_path / f) for f in ("a.bin", "p.bin", "c.bin"))
+    orc.write_binary_coo(pa, A)
+    orc.write_binary_coo(pp, P)
+    meta = drive("galerkin", pa, pp, pc)
+    trpt, tcol, tval = orc.transpose(P)
+    ap = orc.spgemm(A, P, with_abs=False)
+    AP = HostCSR(ap[0], ap[1], ap[2], A.nrows, P.ncols)
+    crpt, ccol, cval, _ = orc.spgemm(HostCSR(trpt, tcol, tval, P.ncols, P.nrows), AP, with_abs=False)
+    assert (meta["rows"], meta["cols"], meta["nnz"]) == (P.ncols, P.ncols, int(crpt[-1]))
+    rpt, col, val, ncols = read_c(pc)
+    np.testing.assert_array_equal(rpt, crpt)
+    np.testing.assert_array_equal(col, ccol)
+    np.testing.assert_allclose(val, cval, rtol=1e-12, atol=1e-12 * np.abs(cval).max())
+
+
+def test_local_prints_reference_summary_and_outtime(orc, tmp_path):
+    """main.cu:241-247: the TOTAL_TIME line on stdout and the outtime.txt
+    line ("%15lu %15lu %15lu " then "%10f %10f %50f\\n")."""
+    import re
+    A = gen.stencil_27pt(8, seed=1)
+    pa, outtime = str(tmp_path / "a.bin"), str(tmp_path / "outtime.txt")
+    orc.write_binary_coo(pa, A)
+    r = subprocess.run([DRIVER, "local", pa, pa, "1", "0", "-", outtime], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    meta = json.loads(r.stdout.strip().splitlines()[-1])
+    assert re.match(r"TOTAL_TIME: \d+\.\d{6}\t\tlocal time: \d+\.\d{6}\t\tnnz = %d\t\tl2norm: \d+\.\d{6}$"
+                    % meta["nnz"], r.stdout.splitlines()[0])
+    line = open(outtime).read()
+    assert line.endswith("\n") and len(line) == 3 * 16 + (10 + 1 + 10 + 1 + 50 + 1)
+    assert line[:48] == "%15d %15d %15d " % (meta["rows"], A.ncols, meta["nnz"])
+    assert abs(float(line[-51:]) - meta["l2norm"]) <= 1e-6 * meta["l2norm"]
