@@ -764,7 +764,7 @@ __global__ __launch_bounds__(kHeavyBS) void k_sym_heavy(int bin_lo, int bin_hi, 
   }
 }
 
-template <typename V, int R>
+template <typename V, int R, int RK>
 __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, int64_t direct_rows,
                                                         const Info *__restrict__ info,
                                                         const int32_t *__restrict__ perm,
@@ -802,11 +802,12 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
     for (int64_t w0 = lo; w0 <= hi; w0 += kNumHeavyWin) {
       for (int q = threadIdx.x; q < NWORD; q += kHeavyBS) bm[q] = 0;
       __syncthreads();
-      row_products<64, R, false, V>(
+      // keys-only walk: RK loads in flight per lane
+      row_products<64, RK, false, V>(
           lane, a0, a1, 64 * w, 64 * kHeavyNW, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[w],
-          [&](const int32_t(&key)[R], const V(&)[R], const bool(&ok)[R], auto &&...) {
+          [&](const int32_t(&key)[RK], const V(&)[RK], const bool(&ok)[RK], auto &&...) {
 #pragma unroll
-            for (int u = 0; u < R; u++) {
+            for (int u = 0; u < RK; u++) {
               const int64_t c = (int64_t)key[u] - w0;
               if (ok[u] && c >= 0 && c < kNumHeavyWin) atomicOr(&bm[c >> 5], 1u << (c & 31));
             }
@@ -955,8 +956,8 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
     case 6: NUM_BLOCK(256, 2048, 2); break;
     case 7: NUM_BLOCK(512, 4096, 2); break;
     case 8:
-      hipLaunchKernelGGL((k_num_heavy<V, kNumR>), dim3(heavy_grid(h, count)), dim3(kHeavyBS), 0, st, lo, bin,
-                         direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval, crpt, ccol, cval);
+      hipLaunchKernelGGL((k_num_heavy<V, kNumR, kSymR>), dim3(heavy_grid(h, count)), dim3(kHeavyBS), 0, st, lo,
+                         bin, direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval, crpt, ccol, cval);
       break;
     default: break;
   }

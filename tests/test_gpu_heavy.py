@@ -82,6 +82,26 @@ def test_heavy_rows_many_windows(nsp, orc, dtype):
     check_against_oracle(A, B, C, dtype, orc)
 
 
+def test_heavy_rows_handle_reuse(nsp, orc):
+    """Heavy-row products on one handle, with different column counts and
+    heavy-row counts in between, all match the oracle."""
+    print("case: heavy handle reuse", flush=True)
+    rng = np.random.default_rng(12)
+    lens = rng.integers(1, 20, 300)
+    lens[[1, 2, 3, 200]] = [450, 380, 520, 300]
+    A1 = rows_matrix(lens, 4000, seed=13)
+    B1 = rows_matrix(rng.integers(40, 100, 4000), 70000, seed=14)
+    lens2 = rng.integers(1, 10, 100)
+    lens2[[9]] = [700]
+    A2 = rows_matrix(lens2, 3000, seed=15)
+    B2 = rows_matrix(rng.integers(30, 80, 3000), 120000, seed=16)
+    h = nsp.Handle()
+    for A, B in ((A1, B1), (A2, B2), (A1, B1), (A2, B2)):
+        C = h.spgemm(nsp.DeviceCSR.from_host(A), nsp.DeviceCSR.from_host(B))
+        assert h.stats()["num_bin"][8] >= 1
+        check_against_oracle(A, B, C, "f64", orc)
+
+
 def test_heavy_rows_in_graph(nsp, orc):
     print("case: heavy graph", flush=True)
     import torch
