@@ -677,7 +677,8 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
 constexpr int kHeavyBS = 1024;
 constexpr int kHeavyNW = kHeavyBS / 64;
 constexpr int64_t kSymHeavyWin = 1 << 20;   // columns per symbolic window (128 KB of bits)
-constexpr int64_t kNumHeavyWin = 1 << 19;   // columns per numeric window (64 KB bits + 64 KB ranks)
+constexpr int64_t kNumHeavyWin = 1 << 20;   // columns per numeric window (128 KB bits + 8 KB group ranks)
+constexpr int kRankGroup = 16;              // bitmap words per stored rank
 
 // Smallest and largest column of a row's products.
 template <int BS>
@@ -779,8 +780,9 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
                                                         V *__restrict__ cval) {
   constexpr int NWORD = (int)(kNumHeavyWin / 32);
   constexpr int WPT = NWORD / kHeavyBS;  // consecutive words per thread in the rank scan
-  __shared__ uint32_t bm[NWORD];
-  __shared__ uint32_t wbase[NWORD];  // row position of the first column of each word
+  static_assert(WPT % kRankGroup == 0, "a thread's words are whole rank groups");
+  __shared__ __align__(16) uint32_t bm[NWORD];
+  __shared__ uint32_t gbase[NWORD / kRankGroup];  // row position of the first column of each word group
   __shared__ EntryScratch<64, V> ent[kHeavyNW];
   __shared__ int64_t s_red[2 * kHeavyNW];
   __shared__ int s_scan[kHeavyNW + 1];
@@ -824,7 +826,7 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
 #pragma unroll 1
       for (int q = 0; q < WPT; q++) {
         uint32_t x = bm[q0 + q];
-        wbase[q0 + q] = (uint32_t)run;
+        if (q % kRankGroup == 0) gbase[(q0 + q) / kRankGroup] = (uint32_t)run;
         while (x) {
           const int b = __ffs(x) - 1;
           x &= x - 1;
@@ -844,8 +846,24 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
             for (int u = 0; u < R; u++) {
               const int64_t c = (int64_t)key[u] - w0;
               if (ok[u] && c >= 0 && c < kNumHeavyWin) {
+                // rank = group base + set bits of the group's words before
+                // this column (four 16-byte LDS reads, masked popcounts)
                 const int q = (int)(c >> 5), b = (int)(c & 31);
-                const int64_t rk = base + wbase[q] + __popc(bm[q] & ((1u << b) - 1u));
+                const int g = q / kRankGroup, j = q % kRankGroup;
+                const uint4 *G = reinterpret_cast<const uint4 *>(&bm[g * kRankGroup]);
+                int cnt = (int)gbase[g];
+#pragma unroll
+                for (int v4 = 0; v4 < kRankGroup / 4; v4++) {
+                  const uint4 x = G[v4];
+                  const uint32_t wv[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                  for (int e = 0; e < 4; e++) {
+                    const int i = v4 * 4 + e;
+                    const uint32_t m = i < j ? 0xffffffffu : (i == j ? ((1u << b) - 1u) : 0u);
+                    cnt += __popc(wv[e] & m);
+                  }
+                }
+                const int64_t rk = base + cnt;
                 if (rk < nzr) unsafeAtomicAdd(&cval[c0 + rk], prod[u]);
               }
             }
