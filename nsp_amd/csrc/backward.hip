@@ -435,7 +435,7 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
   const bool contrib = at_perm && nprod > 0 && (size_t)nprod * sizeof(V) <= kBackwardContribMax;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = nnzA / kScanTile + 2;
-  const size_t o_rows = 256, o_status = o_rows + al((size_t)std::max<int64_t>(MA, 1) * 4),
+  const size_t o_rows = 256, o_status = o_rows + 2 * al((size_t)std::max<int64_t>(MA, 1) * 4),
                o_poff = o_status + al((size_t)ntiles * 8), o_w = o_poff + al((size_t)(nnzA + 1) * 8),
                need = contrib ? o_w + al((size_t)nprod * sizeof(V)) : o_poff;
   if (need > h->bwd.bytes) {
@@ -450,11 +450,13 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
     h->bwd.bytes = need + need / 8;
   }
   char *base = (char *)h->bwd.base;
-  uint32_t *flags = (uint32_t *)base;  // [0] too long, [1] rows for the second tier, [2] scan ticket
-  int32_t *long_rows = (int32_t *)(base + o_rows);
+  // [0] too long, [1] rows for the second tier, [2] scan ticket, [3] rows for the third tier
+  uint32_t *flags = (uint32_t *)base;
+  int32_t *mid_rows = (int32_t *)(base + o_rows);
+  int32_t *long_rows = (int32_t *)(base + o_rows + al((size_t)std::max<int64_t>(MA, 1) * 4));
   int64_t *poff = contrib ? (int64_t *)(base + o_poff) : nullptr;
   V *w = contrib ? (V *)(base + o_w) : nullptr;
-  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 3);
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 4);
   if (contrib && !h->poff_valid) {
     // product offsets of the A entries: exclusive scan of the B row lengths
     uint64_t *status = (uint64_t *)(base + o_status);
@@ -465,15 +467,22 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
     h->poff_valid = true;
   }
   if (MA > 0) {
-    // tier 1: C rows up to 128 entries (32 lanes per row); longer rows go to
-    // a device list that tier 2 (up to 1024 entries, 64 lanes) walks
+    // tier 1: C rows up to 32 entries (32 lanes per row, 64-slot tables);
+    // longer rows go to a device list for tier 2 (up to 128 entries), whose
+    // longer rows go to one for tier 3 (up to 1024 entries, 64 lanes).  The
+    // small first tier keeps the LDS per block low (occupancy) for the
+    // common short C rows.
     const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 8));
-    hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128>), dim3(ga), dim3(256), 0, st, MA, (const int32_t *)nullptr,
+    hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 32>), dim3(ga), dim3(256), 0, st, MA, (const int32_t *)nullptr,
                        (const unsigned int *)nullptr, (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
                        (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
-                       (const int64_t *)poff, w, long_rows, flags + 1, (uint32_t *)nullptr);
+                       (const int64_t *)poff, w, mid_rows, flags + 1, (uint32_t *)nullptr);
+    hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128>), dim3(h->num_cu * 2), dim3(256), 0, st, MA,
+                       (const int32_t *)mid_rows, (const unsigned int *)(flags + 1), (const RowT *)A->row_ptr,
+                       h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
+                       (const V *)A->val, (const int64_t *)poff, w, long_rows, flags + 3, (uint32_t *)nullptr);
     hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024>), dim3(h->num_cu), dim3(256), 0, st, MA,
-                       (const int32_t *)long_rows, (const unsigned int *)(flags + 1), (const RowT *)A->row_ptr,
+                       (const int32_t *)long_rows, (const unsigned int *)(flags + 3), (const RowT *)A->row_ptr,
                        h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
                        (const V *)A->val, (const int64_t *)poff, w, (int32_t *)nullptr, (unsigned int *)nullptr,
                        flags);
