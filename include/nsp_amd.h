@@ -94,11 +94,11 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
 /* Tuning knobs (nsp_amd.autotune picks them per workload):
  *   "grid_cap"     blocks per CU for persistent launches (1..64)
  *   "sym_small_rows_per_wave"  rows per wavefront in the symbolic bins of
- *                  up to 128 products (1, 2, 4)
+ *                  up to 128 products (1, 2, 4, 8)
  *   "sym_rows_per_wave"  rows per wavefront in the symbolic bins of
  *                  129..1024 products (1, 2, 4)
  *   "num_small_rows_per_wave"  rows per wavefront in the numeric bins of up
- *                  to 64 entries (1, 2, 4)
+ *                  to 64 entries (1, 2, 4, 8)
  *   "num_rows_per_wave"  rows per wavefront in the numeric bins of 65..256
  *                  entries (1, 2, 4)
  *   "num_table_factor"   numeric hash slots per row entry (2, 4)

@@ -33,8 +33,8 @@ import statistics
 DEFAULTS = {"sym_small_rows_per_wave": 4, "sym_rows_per_wave": 2, "num_small_rows_per_wave": 4,
             "num_rows_per_wave": 2, "num_table_factor": 2, "grid_cap": 8, "heavy_blocks": 0}
 
-SPACE_SYM = {"sym_small_rows_per_wave": (1, 2, 4), "sym_rows_per_wave": (1, 2, 4)}
-SPACE_NUM = {"num_small_rows_per_wave": (1, 2, 4), "num_rows_per_wave": (1, 2, 4), "num_table_factor": (2, 4)}
+SPACE_SYM = {"sym_small_rows_per_wave": (1, 2, 4, 8), "sym_rows_per_wave": (1, 2, 4)}
+SPACE_NUM = {"num_small_rows_per_wave": (1, 2, 4, 8), "num_rows_per_wave": (1, 2, 4), "num_table_factor": (2, 4)}
 SPACE_GRID = {"grid_cap": (4, 8, 16)}
 
 CACHE_ENV = "NSP_AMD_TUNE_CACHE"

@@ -160,20 +160,21 @@ int nsp_get_stats(nsp_handle_t hs, nsp_stats_t *out) {
 int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
   if (!hs || !name) return NSP_ERR_INVALID;
   Handle *h = &hs->h;
-  auto rpw = [&](int *dst, const char *what) {
-    if (value != 1 && value != 2 && value != 4)
-      return nsp::set_error(h, NSP_ERR_INVALID, std::string(what) + " must be 1, 2 or 4");
+  auto rpw = [&](int *dst, const char *what, bool small) {
+    if (value != 1 && value != 2 && value != 4 && !(small && value == 8))
+      return nsp::set_error(h, NSP_ERR_INVALID,
+                            std::string(what) + (small ? " must be 1, 2, 4 or 8" : " must be 1, 2 or 4"));
     *dst = (int)value;
     return (int)NSP_OK;
   };
   if (!std::strcmp(name, "sym_small_rows_per_wave")) {
-    return rpw(&h->sym_small_rows_per_wave, name);
+    return rpw(&h->sym_small_rows_per_wave, name, true);
   } else if (!std::strcmp(name, "sym_rows_per_wave")) {
-    return rpw(&h->sym_rows_per_wave, name);
+    return rpw(&h->sym_rows_per_wave, name, false);
   } else if (!std::strcmp(name, "num_small_rows_per_wave")) {
-    return rpw(&h->num_small_rows_per_wave, name);
+    return rpw(&h->num_small_rows_per_wave, name, true);
   } else if (!std::strcmp(name, "num_rows_per_wave")) {
-    return rpw(&h->num_rows_per_wave, name);
+    return rpw(&h->num_rows_per_wave, name, false);
   } else if (!std::strcmp(name, "num_table_factor")) {
     if (value != 2 && value != 4) return nsp::set_error(h, NSP_ERR_INVALID, "num_table_factor must be 2 or 4");
     h->num_table_factor = (int)value;

@@ -916,10 +916,13 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
   const int rpw = bin <= 2 ? h->sym_small_rows_per_wave : h->sym_rows_per_wave;
 #define SYM_RPW(SH) \
   if (rpw >= 4) SYM_GROUP(16, SH); else if (rpw == 2) SYM_GROUP(32, SH); else SYM_GROUP(64, SH);
+  // bins 0..2 also take 8 rows per wavefront (8-lane groups)
+#define SYM_RPW_SMALL(SH) \
+  if (rpw >= 8) { SYM_GROUP(8, SH); } else { SYM_RPW(SH); }
   switch (bin) {
-    case 0: SYM_RPW(32); break;
-    case 1: SYM_RPW(64); break;
-    case 2: SYM_RPW(128); break;
+    case 0: SYM_RPW_SMALL(32); break;
+    case 1: SYM_RPW_SMALL(64); break;
+    case 2: SYM_RPW_SMALL(128); break;
     case 3: SYM_RPW(256); break;
     case 4: SYM_RPW(512); break;
     case 5: SYM_RPW(1024); break;
@@ -931,6 +934,7 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
       break;
     default: break;
   }
+#undef SYM_RPW_SMALL
 #undef SYM_RPW
 #undef SYM_GROUP
 #undef SYM_BLOCK
@@ -963,9 +967,12 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
 #define NUM_F(GS, NS) \
   if (f4) NUM_GROUP(GS, NS, 4); else NUM_GROUP(GS, NS, 2);
   switch (bin) {
-    case 0: NUM_F(16, 16); break;
-    case 1: if (rpw >= 4) { NUM_F(16, 32); } else { NUM_F(32, 32); } break;
-    case 2: if (rpw >= 4) { NUM_F(16, 64); } else if (rpw == 2) { NUM_F(32, 64); } else { NUM_F(64, 64); } break;
+    case 0: if (rpw >= 8) { NUM_F(8, 16); } else { NUM_F(16, 16); } break;
+    case 1: if (rpw >= 8) { NUM_F(8, 32); } else if (rpw >= 4) { NUM_F(16, 32); } else { NUM_F(32, 32); } break;
+    case 2:
+      if (rpw >= 8) { NUM_F(8, 64); } else if (rpw >= 4) { NUM_F(16, 64); } else if (rpw == 2) { NUM_F(32, 64); }
+      else { NUM_F(64, 64); }
+      break;
     case 3: if (rpw >= 4) { NUM_F(16, 128); } else if (rpw == 2) { NUM_F(32, 128); } else { NUM_F(64, 128); } break;
     case 4:
       if (rpw >= 4) NUM_GROUP(16, 256, 2); else if (rpw == 2) NUM_GROUP(32, 256, 2); else NUM_GROUP(64, 256, 2);

@@ -77,11 +77,12 @@ def test_all_bins(nsp, orc, dtype):
     check_against_oracle(A, B, C, dtype, orc)
 
 
-@pytest.mark.parametrize("rpw", [1, 2, 4])
+@pytest.mark.parametrize("rpw", [1, 2, 4, 8])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_rows_per_wave_and_table_variants(nsp, orc, rpw, dtype):
     """Every wave-level kernel shape (1, 2 or 4 rows per wavefront in each
-    bin group, table factor 2 and 4) gives the same product."""
+    bin group, 8 in the small bins, table factor 2 and 4) gives the same
+    product."""
     print("case: rows per wave", rpw, dtype, flush=True)
     A = to_dtype(mixed_rows(1500, 1200, 1, 60, seed=3), dtype)
     B = to_dtype(mixed_rows(1200, 1600, 1, 40, seed=4), dtype)
@@ -89,7 +90,7 @@ def test_rows_per_wave_and_table_variants(nsp, orc, rpw, dtype):
         h = nsp.Handle()
         for name in ("sym_small_rows_per_wave", "sym_rows_per_wave", "num_small_rows_per_wave",
                      "num_rows_per_wave"):
-            h.set_param(name, rpw)
+            h.set_param(name, min(rpw, 4) if name in ("sym_rows_per_wave", "num_rows_per_wave") else rpw)
         h.set_param("num_table_factor", tf)
         C, _ = run(nsp, A, B, dtype, handle=h)
         st = h.stats()
