@@ -62,7 +62,8 @@ __global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles, int64_
 // is not needed, so the work is spread evenly whatever the row lengths
 // (power-law matrices put most entries in a few rows).
 //
-// k_rowinfo (16 lanes per row): intermediate products per row as the sum of
+// k_rowinfo (one lane per row, 8 loads in flight; rows past kLongRow
+// entries go to k_rowinfo_long): intermediate products per row as the sum of
 // its entries' ab_len (nsp.cu:306-320), the symbolic bin, per-bin histogram,
 // nprod, max, and an int64 copy of A's row pointer.
 //
@@ -181,29 +182,33 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
                                                  int64_t *__restrict__ arpt64, int64_t *__restrict__ intprod,
                                                  int32_t *__restrict__ row_nz, uint8_t *__restrict__ cls,
                                                  int32_t *__restrict__ long_rows, Info *info) {
-  constexpr int GS = 16;
   __shared__ unsigned long long hist[kBins];
   __shared__ unsigned long long s_nprod, s_max;
   if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
   if (threadIdx.x == 0) { s_nprod = 0; s_max = 0; }
   __syncthreads();
-  const int glane = threadIdx.x % GS;
   unsigned long long my_sum = 0, my_max = 0;
-  for (int64_t i = blockIdx.x * (int64_t)(256 / GS) + threadIdx.x / GS; i < MA;
-       i += (int64_t)gridDim.x * (256 / GS)) {
+  // one lane per row, its B row lengths loaded 8 at a time (independent
+  // loads in flight), so a wavefront has 64 rows in flight and the per-row
+  // outputs are written coalesced
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < MA; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
     if (p1 - p0 > kLongRow) {  // k_rowinfo_long finishes it
-      if (glane == 0) long_rows[atomicAdd((unsigned long long *)&info->nlong, 1ull)] = (int32_t)i;
+      long_rows[atomicAdd((unsigned long long *)&info->nlong, 1ull)] = (int32_t)i;
       continue;
     }
-    int64_t s = 0;
-    for (int64_t p = p0 + glane; p < p1; p += GS) s += ab_len[p];
-    s = group_sum<GS>(s);
-    if (glane == 0) {
-      finish_row(i, (unsigned long long)s, p0, intprod, arpt64, row_nz, cls, hist);
-      my_sum += (unsigned long long)s;
-      my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
+    int64_t s = 0, p = p0;
+    for (; p + 8 <= p1; p += 8) {
+      int32_t l[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) l[u] = ab_len[p + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += l[u];
     }
+    for (; p < p1; p++) s += ab_len[p];
+    finish_row(i, (unsigned long long)s, p0, intprod, arpt64, row_nz, cls, hist);
+    my_sum += (unsigned long long)s;
+    my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) arpt64[MA] = (int64_t)arpt[MA];
   my_sum = group_sum<64>(my_sum);
@@ -1130,7 +1135,7 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
     hipLaunchKernelGGL(k_extents<RowT>, dim3(ge), dim3(256), 0, h->stream, nnzA, (const RowT *)A->row_ptr, A->col,
                        (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->ab_b0, h->ab_len);
   }
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 15) / 16, (int64_t)h->num_cu * 8));
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 255) / 256, (int64_t)h->num_cu * 8));
   // perm_num is free until the numeric binning: it holds the long-row list
   hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr, h->ab_len,
                      h->a_rpt64, h->intprod, h->row_nz, h->cls, h->perm_num, h->d_info);

@@ -60,8 +60,8 @@ def test_heavy_rows_one_window(nsp, orc, dtype, index):
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_heavy_rows_many_windows(nsp, orc, dtype):
     """B has 3.2M columns and its rows are spread over all of them, so heavy
-    C rows span several symbolic (1M-column) and numeric (512K-column)
-    windows; one row's products all land in one narrow band."""
+    C rows span several 2^20-column windows (symbolic and numeric); one
+    row's products all land in one narrow band."""
     print("case: heavy many windows", dtype, flush=True)
     rng = np.random.default_rng(4)
     ncols = 3_200_000
