@@ -98,3 +98,27 @@ def test_galerkin_shape_errors(h):
     P = nsp_amd.DeviceCSR.from_host(gen.random_csr(20, 5, 0.2, seed=2))
     with pytest.raises(NspError):
         h.galerkin(A, P)
+
+
+def test_galerkin_plan_relaunch_with_new_values(orc):
+    """GalerkinPlan (two recorded graphs, P^T values by gather): launches
+    after the values of A and P change in place match the oracle's products
+    of the new values; the patterns stay."""
+    import torch
+    import nsp_amd
+    from nsp_amd.galerkin import GalerkinPlan
+    A = gen.stencil_27pt(12, seed=5)
+    P = smoothed(orc, A, gen.aggregation(A.nrows, seed=6))
+    dA, dP = nsp_amd.DeviceCSR.from_host(A), nsp_amd.DeviceCSR.from_host(P)
+    plan = GalerkinPlan(dA, dP)
+    rng = np.random.default_rng(7)
+    for step in range(3):
+        print(f"  plan launch {step}", flush=True)
+        plan.launch()
+        C = plan.result()
+        check(C, oracle_galerkin(orc, transpose_host(orc, P), A, P), "f64", orc)
+        # new values, same patterns
+        A = HostCSR(A.rpt, A.col, A.val * rng.uniform(0.5, 1.5, A.nnz), A.nrows, A.ncols)
+        P = HostCSR(P.rpt, P.col, P.val * rng.uniform(0.5, 1.5, P.nnz), P.nrows, P.ncols)
+        dA.val.copy_(torch.from_numpy(A.val))
+        dP.val.copy_(torch.from_numpy(P.val))
