@@ -65,13 +65,14 @@ def algorithmic_bytes(A, B, nnzC, s=8):
     return sym, scan, num
 
 
-TRAFFIC_FILES = {"c2": "spgemm_numeric_stencil128_rank0_traffic.json"}
+TRAFFIC_FILES = {"c2": "spgemm_numeric_stencil128_rank0_traffic.json",
+                 "coo": "coo_read_stencil128_rank0_traffic.json"}
 
 
 def measured_traffic(workload: str):
-    """HBM bytes per launch of the numeric kernel from the committed counter
-    summary (FETCH_SIZE and WRITE_SIZE passes scaled by a plain copy's), or
-    None when the workload has none."""
+    """HBM bytes per launch of a workload's dominant kernel from the
+    committed counter summary (FETCH_SIZE and WRITE_SIZE passes scaled by a
+    plain copy's), or None when the workload has none."""
     name = TRAFFIC_FILES.get(workload)
     path = os.path.join(ROOT, "profiles", name) if name else None
     if not path or not os.path.exists(path):
@@ -378,7 +379,7 @@ def coo_leg(A, B, steps: int = 10, reps: int = 3):
             "kernel_ms": round(kernel_ms, 4),
             "roofline": {"bound": "hbm", "kernel": "k_coo_chunk (records -> CSR)", "achieved": round(achieved, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                         "algorithmic_bytes": alg},
+                         "traffic": measured_traffic("coo"), "algorithmic_bytes": alg},
             "file_to_device_csr_s": round(e2e_s, 4),
             "file_GBs": round(file_bytes / e2e_s / 1e9, 2),
             "cpu_reader_s": round(cpu_s, 4), "cpu_reader_kind": "port (oracle, one thread)",
