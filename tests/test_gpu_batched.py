@@ -174,3 +174,53 @@ def test_batched_with_empty_matrices_and_rows(nsp, orc):
     dX = nsp.DeviceCSR.from_host(X)
     C = h.batched_spgemm(dX, dev_off(off), dX, dev_off(off))
     check_against_oracle(X, X, C, "f64", orc, ref=oracle_batch(orc, X, off))
+
+
+def test_batched_full_size_4096(nsp, orc):
+    """The batched workload at full size (4096 uneven FP32 products,
+    8.58 M rows): one graph launch; 64 seeded matrices against the oracle;
+    C x = A (A x) over every row of the block-diagonal batch (SpMV on the
+    host in FP64, relative to |A| (|A| |x|), 1e-5 for FP32 values)."""
+    import scipy.sparse as sp
+    import torch
+    from tests.helpers import TOL
+    print("case: batched 4096 full size", flush=True)
+    X, off = gen.uneven_batch_concat(4096, seed=0)
+    X = to_dtype(X, "f32")
+    d_off = dev_off(off)
+    dX = nsp.DeviceCSR.from_host(X, "f32")
+    h = nsp.Handle()
+    _, nnz = h.batched_symbolic(dX, d_off, dX, d_off)
+    g = h.graph(dX, dX, capacity=nnz, a_off=d_off, b_off=d_off)
+    g.launch()
+    C = g.result()
+    rpt = C.row_ptr.cpu().numpy().astype(np.int64)
+    col = C.col.cpu().numpy()
+    val = C.val.cpu().numpy().astype(np.float64)
+    # 64 seeded matrices against the oracle
+    worst = 0.0
+    for b in np.random.default_rng(3).choice(4096, 64, replace=False):
+        r0, r1 = int(off[b]), int(off[b + 1])
+        m = X.row_slice(r0, r1)
+        m.ncols = r1 - r0
+        crpt, ccol, cval, cabs = orc.spgemm(m, m)
+        s0 = rpt[r0]
+        np.testing.assert_array_equal(rpt[r0:r1 + 1] - s0, crpt)
+        np.testing.assert_array_equal(col[s0:rpt[r1]], ccol)
+        comp, _ = orc.componentwise_error(val[s0:rpt[r1]], cval, cabs)
+        worst = max(worst, comp)
+    assert worst <= TOL["f32"], worst
+    # block-diagonal identity over all rows: global column = local + matrix offset
+    mat_of_row = np.repeat(np.arange(4096), np.diff(off))
+    shift_c = np.repeat(off[:-1][mat_of_row], np.diff(rpt))
+    Cs = sp.csr_matrix((val, col.astype(np.int64) + shift_c, rpt), shape=(X.nrows, X.nrows))
+    shift_a = np.repeat(off[:-1][mat_of_row], np.diff(X.rpt))
+    As = sp.csr_matrix((X.val, X.col.astype(np.int64) + shift_a, X.rpt), shape=(X.nrows, X.nrows))
+    x = np.random.default_rng(4).uniform(-1, 1, X.nrows)
+    y1, y2 = Cs @ x, As @ (As @ x)
+    scale = abs(As) @ (abs(As) @ np.abs(x))
+    rel = float(np.max(np.abs(y1 - y2) / np.where(scale > 0, scale, 1.0)))
+    print(f"  nnz {C.nnz}, 64 matrices err {worst:.2e}, C x vs A(A x) err {rel:.2e}", flush=True)
+    assert rel <= TOL["f32"], rel
+    del C, g
+    torch.cuda.empty_cache()
