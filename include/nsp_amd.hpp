@@ -17,6 +17,7 @@
  *   load_MatrixMPI                    src/distribute.cu:1071-1125 (device CSR)
  *   write_binary_matrix               main.cu:468-512
  *   CSRMatrixPrintMM                  src/matrixIO.cu:299-339
+ *   CSRm::printInfo                   src/CSR.cu:631-643
  *   append_outtime                    main.cu:236-247 (the outtime.txt line)
  *   galerkin_product                  R(AP), named at main.cu:138
  * Errors: where the reference exits (checkCudaErrors, helper_cuda.h:714) these
@@ -29,6 +30,7 @@
 #define NSP_AMD_HPP
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 
@@ -112,6 +114,10 @@ CSR *copyToDevice(const CSR *A);
 CSR *copyToHost(const CSR *A);
 /* ||A||_F of a host or device matrix (main.cu:236-241 prints it per rank) */
 double norm_fro(const CSR *A);
+/* The reference's summary of a matrix (src/CSR.cu:631-643, printed by
+ * main.cu's printinfo option), same lines and formats; this CSR carries no
+ * symmetry flag, so the last line is GENERAL as for every loaded matrix. */
+void printInfo(const CSR *A, FILE *fp = stdout);
 }  // namespace CSRm
 
 /* Compacts A's columns once (src/spspmpi.cu:141-162): perm = sorted

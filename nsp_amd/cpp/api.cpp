@@ -172,6 +172,18 @@ double norm_fro(const CSR *A) {
   return std::sqrt(s);
 }
 
+void printInfo(const CSR *A, FILE *fp) {
+  // src/CSR.cu:631-643 (integers as the reference prints them)
+  fprintf(fp, "Device?: %d\n", A->on_the_device ? 1 : 0);
+  fprintf(fp, "nnz: %d\n", A->nnz);
+  fprintf(fp, "n: %d\n", A->n);
+  fprintf(fp, "m: %lld\n", (long long)A->m);
+  fprintf(fp, "full_n: %lld\n", (long long)A->full_n);
+  fprintf(fp, "row_shift: %lld\n", (long long)A->row_shift);
+  fprintf(fp, "GENERAL\n");
+  fprintf(fp, "\n");
+}
+
 }  // namespace CSRm
 
 // ---------------------------------------------------------------------------

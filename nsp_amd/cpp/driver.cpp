@@ -7,7 +7,9 @@
 //   nsp_amd_run local A.bin P.bin nprocs rank [C.bin|-] [outtime.txt]
 //       rank `rank` of `nprocs`: its row block of A times the P rows it needs
 //       (nsparseMGPU_version1 without MPI: P is read whole onto the device);
-//       prints main.cu's TOTAL_TIME line and appends its outtime.txt line
+//       prints main.cu's TOTAL_TIME line and appends its outtime.txt line;
+//       NSP_AMD_PRINTINFO=1 prints both matrices' summaries first
+//       (CSRm::printInfo, main.cu's printinfo option)
 //   nsp_amd_run split A.bin nprocs rank out.bin
 //       the reader/splitter alone (host only): the rank's block, columns in
 //       the rank frame, written back as binary COO
@@ -91,6 +93,10 @@ static int run_local(int argc, char **argv) {
   CSR *Ad = load_MatrixMPI(argv[2], nprocs, rank);
   CSR *Pd = load_MatrixMPI(argv[3], 1, 0);
   const double read_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (getenv("NSP_AMD_PRINTINFO")) {  // main.cu's printinfo option
+    CSRm::printInfo(Ad);
+    CSRm::printInfo(Pd);
+  }
   nsp_coo_share_t pshare;
   if (nsp_coo_share(argv[3], 1, nprocs, rank, &pshare) != NSP_OK) throw Error(NSP_ERR_INVALID, "P share");
   double loc_time = 0.0;

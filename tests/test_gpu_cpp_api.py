@@ -132,11 +132,19 @@ def test_local_prints_reference_summary_and_outtime(orc, tmp_path):
     A = gen.stencil_27pt(8, seed=1)
     pa, outtime = str(tmp_path / "a.bin"), str(tmp_path / "outtime.txt")
     orc.write_binary_coo(pa, A)
-    r = subprocess.run([DRIVER, "local", pa, pa, "1", "0", "-", outtime], capture_output=True, text=True, timeout=120)
+    env = dict(os.environ, NSP_AMD_PRINTINFO="1")
+    r = subprocess.run([DRIVER, "local", pa, pa, "1", "0", "-", outtime], capture_output=True, text=True, timeout=120,
+                       env=env)
     assert r.returncode == 0, r.stderr
     meta = json.loads(r.stdout.strip().splitlines()[-1])
+    # CSRm::printInfo of A and P (src/CSR.cu:631-643), device matrices
+    info = r.stdout.split("TOTAL_TIME")[0]
+    n = A.nrows
+    block = f"Device?: 1\nnnz: {A.nnz}\nn: {n}\nm: {n}\nfull_n: {n}\nrow_shift: 0\nGENERAL\n\n"
+    assert info == block + block, info
+    total_line = [l for l in r.stdout.splitlines() if l.startswith("TOTAL_TIME")][0]
     assert re.match(r"TOTAL_TIME: \d+\.\d{6}\t\tlocal time: \d+\.\d{6}\t\tnnz = %d\t\tl2norm: \d+\.\d{6}$"
-                    % meta["nnz"], r.stdout.splitlines()[0])
+                    % meta["nnz"], total_line)
     line = open(outtime).read()
     assert line.endswith("\n") and len(line) == 3 * 16 + (10 + 1 + 10 + 1 + 50 + 1)
     assert line[:48] == "%15d %15d %15d " % (meta["rows"], A.ncols, meta["nnz"])
