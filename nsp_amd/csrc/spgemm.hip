@@ -531,7 +531,7 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
                                                    const V *__restrict__ bval,
                                                    const int64_t *__restrict__ crpt,
                                                    int32_t *__restrict__ ccol,
-                                                   V *__restrict__ cval) {
+                                                   V *__restrict__ cval, int pack) {
   constexpr int RPB = 256 / GS;
   constexpr int SH = NS * SHF;    // table slots (NS = largest row nnz of the bin)
   constexpr int LOG = Log2<SH>::value;
@@ -569,7 +569,9 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
                                      hash_accumulate_batch<LOG, R, Acc>(K, W, key, pa, ok, glane);
                                    });
       wave_lds_sync();
-      // compact occupied slots (strided: slot q*GS + glane) into S
+      // compact occupied slots (strided: slot q*GS + glane) into S; with
+      // `pack` (C's columns below 2^(32-LOG)) the key carries its slot in
+      // the low LOG bits, so the sorted entries read their values directly
       int c = 0;
 #pragma unroll
       for (int q = 0; q < SPL; q++) c += (K[q * GS + glane] != kEmpty);
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
 #pragma unroll
       for (int q = 0; q < SPL; q++) {
         const int32_t k = K[q * GS + glane];
-        if (k != kEmpty) S[pos++] = (uint32_t)k;
+        if (k != kEmpty) S[pos++] = pack ? (((uint32_t)k << LOG) | (uint32_t)(q * GS + glane)) : (uint32_t)k;
       }
       wave_lds_sync();
       const int64_t c0 = crpt[r];
@@ -589,12 +591,23 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
         v[q] = (g < nz) ? S[g] : kPadKey;
       }
       group_bitonic_sort<GS, E>(v, glane);
+      if (pack) {
 #pragma unroll
-      for (int q = 0; q < E; q++) {
-        const int g = glane * E + q;
-        if (g < nz) {
-          ccol[c0 + g] = (int32_t)v[q];
-          cval[c0 + g] = (V)hash_lookup<LOG, Acc>(K, W, (int32_t)v[q]);
+        for (int q = 0; q < E; q++) {
+          const int g = glane * E + q;
+          if (g < nz) {
+            ccol[c0 + g] = (int32_t)(v[q] >> LOG);
+            cval[c0 + g] = (V)W[v[q] & ((1u << LOG) - 1u)];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+          const int g = glane * E + q;
+          if (g < nz) {
+            ccol[c0 + g] = (int32_t)v[q];
+            cval[c0 + g] = (V)hash_lookup<LOG, Acc>(K, W, (int32_t)v[q]);
+          }
         }
       }
       wave_lds_sync();
@@ -945,11 +958,19 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
 #undef SYM_BLOCK
 }
 
+// Whether C's columns leave room for a table slot of `slots` entries in the
+// low bits of a 32-bit sort key (and stay below the padding key).
+static int pack_ok(int64_t ncols, int slots) {
+  int log = 0;
+  while ((1 << log) < slots) log++;
+  return ncols < (int64_t(1) << (32 - log)) ? 1 : 0;
+}
+
 template <typename V>
 static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
                            const int64_t *arpt, const int64_t *ab_b0, const V *aval,
                            const int32_t *ab_len, const int32_t *bcol, const V *bval,
-                           int32_t *ccol, V *cval) {
+                           int32_t *ccol, V *cval, int64_t ncols) {
   hipStream_t st = h->stream;
   const Info *info = h->d_info;
   const int32_t *perm = direct >= 0 ? nullptr : h->perm_num;
@@ -958,7 +979,7 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
 #define NUM_GROUP(GS, NS, SHF)                                                                      \
   hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR>), dim3(grid_for(h, count, 256 / GS, cap)),      \
                      dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,      \
-                     bcol, bval, crpt, ccol, cval)
+                     bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF))
 #define NUM_BLOCK(BS, NS, SHF)                                                                      \
   hipLaunchKernelGGL((k_num_block<V, BS, NS, SHF, kNumR>), dim3(grid_for(h, count, 1, cap)), dim3(BS),   \
                      0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval,     \
@@ -1236,7 +1257,7 @@ static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
   timing_mark(h, 4);
   for (int q = 0; q < n; q++)
     launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, (const V *)A->val,
-                      h->ab_len, B->col, (const V *)B->val, C->col, (V *)C->val);
+                      h->ab_len, B->col, (const V *)B->val, C->col, (V *)C->val, B->cols);
   h->stats.num_launches = n;
   h->stats.direct_mode = (h->stats.direct_mode & 1) | ((n == 1 && L[0].direct >= 0) ? 2 : 0);
   timing_mark(h, 5);
@@ -1312,7 +1333,7 @@ static int graph_record(Handle *g, const nsp_csr_t *A, const nsp_csr_t *B, const
   launch_scatter(g, MA, g->d_info->count_num, g->d_info->cursor_num, g->perm_num);
   for (int b = 0; b < kBins; b++)
     launch_num_bin<V>(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len,
-                      B->col, (const V *)B->val, C->col, (V *)C->val);
+                      B->col, (const V *)B->val, C->col, (V *)C->val, B->cols);
   NSP_HIP(g, hipGetLastError());
   return NSP_OK;
 }

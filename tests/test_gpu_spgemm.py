@@ -214,3 +214,33 @@ def test_stencil_rank0_share_identity(nsp, orc):
     crpt, ccol, cval, cabs = ref
     np.testing.assert_array_equal(C.row.cpu().numpy().astype(np.int64), crpt)
     np.testing.assert_array_equal(C.col.cpu().numpy(), ccol)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_wide_columns_unpacked_keys(nsp, orc, dtype):
+    """C rows of 257-512 entries (numeric tables of 1024 slots) over 5 M
+    columns: past 2^22 columns the table slot no longer fits beside the
+    column in a 32-bit sort key, so these rows take the path that looks
+    their values up by column; rows of smaller bins still pack."""
+    print("case: wide columns", dtype, flush=True)
+    rng = np.random.default_rng(21)
+    ncols = 5_000_000
+    nA, nB = 600, 1200
+    rows, cols = [], []
+    for i in range(nA):
+        k = 10 if i % 3 else int(rng.integers(1, 4))
+        rows.append(np.full(k, i))
+        cols.append(rng.choice(nB, k, replace=False))
+    A = gen._from_rows_cols(nA, nB, np.concatenate(rows), np.concatenate(cols),
+                            rng.uniform(-1, 1, sum(len(c) for c in cols)))
+    rows, cols = [], []
+    for i in range(nB):
+        rows.append(np.full(40, i))
+        cols.append(rng.choice(ncols, 40, replace=False))
+    B = gen._from_rows_cols(nB, ncols, np.concatenate(rows), np.concatenate(cols), rng.uniform(-1, 1, 40 * nB))
+    A, B = to_dtype(A, dtype), to_dtype(B, dtype)
+    h = nsp.Handle()
+    C = h.spgemm(nsp.DeviceCSR.from_host(A, dtype), nsp.DeviceCSR.from_host(B, dtype))
+    st = h.stats()
+    assert st["num_bin"][5] > 0, st["num_bin"]
+    check_against_oracle(A, B, C, dtype, orc)
