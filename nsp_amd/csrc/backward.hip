@@ -220,8 +220,7 @@ int transpose(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *X, nsp
 // of two >= 2 nz, at least 16 slots), so short rows clear and probe small
 // tables.  Same 24-bit multiplicative hash as the forward tables.
 __device__ __forceinline__ uint32_t hash_slot_rt(int32_t key, int log) {
-  const uint32_t p = (uint32_t)__umul24((uint32_t)key, 0x9E3779u);
-  return (p & 0xFFFFFFu) >> (24 - log);
+  return hash_mul24(key) >> (24 - log);
 }
 
 template <typename V>

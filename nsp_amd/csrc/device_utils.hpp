@@ -42,10 +42,18 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 // 1, m, m^2) spread over the table instead of aliasing as under key & (SH-1).
 // Only the low 24 bits of a column enter the hash; columns 2^24 apart collide,
 // which costs probes, never correctness.
+// The multiply is written as the instruction itself: with only the low 24
+// bits of the product used, hipcc otherwise drops __umul24's operand mask and
+// emits the quarter-rate v_mul_lo_u32 (same low bits, four times the cycles).
+__device__ __forceinline__ uint32_t hash_mul24(int32_t key) {
+  uint32_t p;
+  asm("v_mul_u32_u24 %0, 0x9e3779, %1" : "=v"(p) : "v"(key));
+  return p & 0xFFFFFFu;
+}
+
 template <int LOG>
 __device__ __forceinline__ uint32_t hash_slot(int32_t key) {
-  const uint32_t p = (uint32_t)__umul24((uint32_t)key, 0x9E3779u);  // __umul24 returns int
-  return (p & 0xFFFFFFu) >> (24 - LOG);
+  return hash_mul24(key) >> (24 - LOG);
 }
 
 constexpr int32_t kDummy = -2;  // key of a lane's private dummy slot
@@ -123,8 +131,9 @@ __device__ __forceinline__ void hash_accumulate_batch(int32_t *keys, V *vals, co
     if (old[u] != kEmpty && old[u] != kk[u]) h[u] = hash_probe_from<LOG>(keys, kk[u], h[u], ins);
   }
 #pragma unroll
+  // lanes with ok[u] false add into their dummy slot, which no reader visits
   for (int u = 0; u < R; u++)
-    __hip_atomic_fetch_add(&vals[h[u]], ok[u] ? prod[u] : (V)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&vals[h[u]], prod[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---- group (sub-wave) collectives; GS lanes, GS a power of two <= 64 ------

@@ -1,5 +1,5 @@
 // Enumeration of a CSR row's intermediate products a_ik * b_kj by a group of
-// GS lanes (GS = 16, 32 or 64), built for memory-level parallelism on gfx950.
+// GS lanes (GS = 8, 16, 32 or 64), built for memory-level parallelism on gfx950.
 //
 // A entries are taken GS at a time, one per lane.  Each entry's B row start
 // and length come from arrays the binning pass already gathered (ab_b0,
@@ -8,12 +8,14 @@
 // in a window lane g handles products w + u*GS + g, u < R.  Windows are
 // software-pipelined: the B loads of window i+1 are issued before the
 // callback of window i runs, so loads stay in flight while the LDS hash work
-// of the previous window executes.  A product's entry is found without a
-// search: a bit mask marks where entries start, one 64-bit word per GS
-// products, and
-//     entry = #(entries starting before the word) + popc(word & lanes<=g) - 1
-// (entries with empty B rows are compacted away first).  The masks cover a
-// super-window of kMaskWords*GS products at a time.
+// of the previous window executes (two windows alternate roles, so nothing is
+// copied between them).  A product's entry is found without a search: a bit
+// mask marks where entries start, one word per GS products, and
+//     entry = pre[word] + popc(word & lanes<=g) - 1
+// where pre[word] counts the entries starting before the word (entries with
+// empty B rows are compacted away first).  Masks and counts cover a
+// super-window of kMaskWords*GS products at a time; for GS <= 32 the count
+// sits in the upper half of the mask word, so one 64-bit LDS read gives both.
 #pragma once
 #include "device_utils.hpp"
 
@@ -26,7 +28,8 @@ template <int GS, typename V>
 struct EntryScratch {
   int64_t base[GS];          // B index of entry e's first product minus that product's number
   V av[GS];                  // a_ik of entry e
-  unsigned long long mask[kMaskWords];
+  unsigned long long mask[kMaskWords];  // entry starts; GS <= 32: count before the word in bits 32..63
+  int pre[GS == 64 ? kMaskWords : 1];   // GS = 64: count of entries starting before the word
 };
 
 template <int GS>
@@ -91,11 +94,35 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
     }
     for (int sw = 0; sw < T; sw += GS * kMaskWords) {
       // entry-start masks of this super-window
-      if (glane < kMaskWords) es->mask[glane] = 0;
-      if (GS < kMaskWords && glane + GS < kMaskWords) es->mask[glane + GS] = 0;
+      for (int w = glane; w < kMaskWords; w += GS) es->mask[w] = 0;
       wave_lds_sync();
       if (len > 0 && start >= sw && start < sw + GS * kMaskWords)
         atomicOr(&es->mask[(start - sw) / GS], 1ull << ((start - sw) % GS));
+      wave_lds_sync();
+      // count of entries starting before each word: entries before the
+      // super-window plus a group scan of the words' popcounts (WPL words
+      // per lane, in order)
+      {
+        constexpr int WPL = kMaskWords > GS ? kMaskWords / GS : 1;
+        const bool has = glane * WPL < kMaskWords;
+        int cw[WPL], cnt = 0;
+#pragma unroll
+        for (int w = 0; w < WPL; w++) {
+          cw[w] = has ? __popcll(es->mask[glane * WPL + w]) : 0;
+          cnt += cw[w];
+        }
+        int run = __popcll(group_ballot<GS>(len > 0 && start < sw)) + group_incl_scan<GS>(cnt, glane) - cnt;
+        if (has) {
+#pragma unroll
+          for (int w = 0; w < WPL; w++) {
+            if constexpr (GS == 64)
+              es->pre[glane * WPL + w] = run;
+            else
+              es->mask[glane * WPL + w] |= (unsigned long long)(uint32_t)run << 32;
+            run += cw[w];
+          }
+        }
+      }
       wave_lds_sync();
       const int tend = (T - sw < GS * kMaskWords) ? T - sw : GS * kMaskWords;
       const int nwin = (tend + GS * R - 1) / (GS * R);
@@ -106,24 +133,35 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
 #pragma unroll
         for (int u = 0; u < R; u++) {
           const int q = i * R + u;
-          const int before = __popcll(group_ballot<GS>(len > 0 && start < sw + q * GS));
+          const unsigned long long m = es->mask[q];
+          int e;
+          if constexpr (GS == 64)
+            e = es->pre[q] + __popcll(m & le) - 1;
+          else
+            e = (int)(m >> 32) + __popc((uint32_t)m & (uint32_t)le) - 1;
           W.ok[u] = q * GS + glane < tend;
-          const int e = W.ok[u] ? before + __popcll(es->mask[q] & le) - 1 : 0;
+          e = W.ok[u] ? e : 0;
           const int64_t jb = es->base[e] + (W.ok[u] ? sw + q * GS + glane : 0);
           W.key[u] = bcol[jb];
           if (VAL || LOADB) W.bv[u] = bval[jb];
           W.e[u] = e;
         }
       };
-      Window<R, V> cur, nxt;
-      load(0, cur);
-      for (int i = 0; i < nwin; i++) {
-        if (i + 1 < nwin) load(i + 1, nxt);
+      auto consume = [&](Window<R, V> &W) {
         V prod[R];
 #pragma unroll
-        for (int u = 0; u < R; u++) prod[u] = VAL ? es->av[cur.e[u]] * cur.bv[u] : (V)0;
-        f(cur.key, prod, cur.ok, cur.bv, cur.e);
-        cur = nxt;
+        for (int u = 0; u < R; u++) prod[u] = VAL ? es->av[W.e[u]] * W.bv[u] : (V)0;
+        f(W.key, prod, W.ok, W.bv, W.e);
+      };
+      Window<R, V> wa, wb;
+      load(0, wa);
+      for (int i = 0; i < nwin; i += 2) {
+        if (i + 1 < nwin) load(i + 1, wb);
+        consume(wa);
+        if (i + 1 < nwin) {
+          if (i + 2 < nwin) load(i + 2, wa);
+          consume(wb);
+        }
       }
     }
     wave_lds_sync();

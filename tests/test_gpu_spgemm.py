@@ -244,3 +244,35 @@ def test_wide_columns_unpacked_keys(nsp, orc, dtype):
     st = h.stats()
     assert st["num_bin"][5] > 0, st["num_bin"]
     check_against_oracle(A, B, C, dtype, orc)
+
+
+@pytest.mark.parametrize("rpw", [8, 4])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_small_output_rows_with_many_products(nsp, orc, rpw, dtype):
+    """Rows of 100-900 products that collapse onto at most 48 columns (the
+    small numeric bins) with 8 and 4 rows per wave (groups of 8 and 16 lanes): each group walks several
+    mask words and super-windows of products, so every word's entry-start
+    mask and count must be rebuilt for each super-window and row."""
+    print("case: small-output rows, many products, rows per wave", rpw, dtype, flush=True)
+    rng = np.random.default_rng(31)
+    nA, nB = 2000, 300
+    pool = rng.choice(5000, 48, replace=False)
+    rows, cols = [], []
+    for i in range(nB):
+        rows.append(np.full(20, i))
+        cols.append(rng.choice(pool, 20, replace=False))
+    B = gen._from_rows_cols(nB, 5000, np.concatenate(rows), np.concatenate(cols), rng.uniform(-1, 1, 20 * nB))
+    rows, cols = [], []
+    for i in range(nA):
+        k = int(rng.integers(5, 46))
+        rows.append(np.full(k, i))
+        cols.append(rng.choice(nB, k, replace=False))
+    A = gen._from_rows_cols(nA, nB, np.concatenate(rows), np.concatenate(cols),
+                            rng.uniform(-1, 1, sum(len(c) for c in cols)))
+    A, B = to_dtype(A, dtype), to_dtype(B, dtype)
+    h = nsp.Handle()
+    h.set_param("num_small_rows_per_wave", rpw)
+    C, _ = run(nsp, A, B, dtype, handle=h)
+    st = h.stats()
+    assert st["num_bin"][2] > 0 and sum(st["num_bin"][3:]) == 0, st["num_bin"]
+    check_against_oracle(A, B, C, dtype, orc)
