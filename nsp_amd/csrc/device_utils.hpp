@@ -208,35 +208,47 @@ __device__ __forceinline__ int shfl_xor_const(int X, int v) {
 // ---- register bitonic sort over a group ------------------------------------
 // GS lanes hold E keys each (blocked layout: element g = glane*E + r); sorts
 // the GS*E keys ascending.  GS and E powers of two; all GS lanes active.
+// Phase k merges runs of k keys, ascending where (g & k) == 0 and descending
+// elsewhere.  Keys of descending runs are held complemented (~x reverses the
+// order of uint32) for the whole phase, so every compare-exchange puts the
+// minimum at the lower index: an in-lane pair is one min and one max, a
+// cross-lane one a min, a max and a select, with no per-element direction.
+// Leaving phase k and entering phase 2k is one xor per key.
 template <int GS, int E>
 __device__ __forceinline__ void group_bitonic_sort(uint32_t (&v)[E], int glane) {
   constexpr int N = GS * E;
+  // The lane masks below depend on the lane only; an opaque copy of the lane
+  // index keeps the compiler from hoisting them out of the caller's row loop,
+  // where they would stay live (registers) across the whole kernel.
+  int gl = glane;
+  asm volatile("" : "+v"(gl));
+  // complement mask of element gl*E + r in phase kk (none in the last)
+  auto flip = [&](int kk, int r) -> uint32_t {
+    return (kk >= 2 && kk < N && ((gl * E + r) & kk)) ? ~0u : 0u;
+  };
 #pragma unroll
   for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int r = 0; r < E; r++) v[r] ^= flip(k, r) ^ flip(k >> 1, r);
 #pragma unroll
     for (int j = k >> 1; j > 0; j >>= 1) {
       if (j < E) {
 #pragma unroll
         for (int r = 0; r < E; r++) {
           if ((r & j) == 0) {
-            const int g = glane * E + r;
-            const bool asc = (g & k) == 0;
-            uint32_t a = v[r], b = v[r | j];
-            bool sw = asc ? (a > b) : (a < b);
-            v[r] = sw ? b : a;
-            v[r | j] = sw ? a : b;
+            const uint32_t a = v[r], b = v[r | j];
+            v[r] = a < b ? a : b;
+            v[r | j] = a < b ? b : a;
           }
         }
       } else {
+        const bool lower = ((gl * E) & j) == 0;
 #pragma unroll
         for (int r = 0; r < E; r++) {
-          const int g = glane * E + r;
-          uint32_t o = (uint32_t)shfl_xor_const(j / E, (int)v[r]);
-          const bool lower = (g & j) == 0;
-          const bool asc = (g & k) == 0;
-          uint32_t mn = v[r] < o ? v[r] : o;
-          uint32_t mx = v[r] < o ? o : v[r];
-          v[r] = (lower == asc) ? mn : mx;
+          const uint32_t o = (uint32_t)shfl_xor_const(j / E, (int)v[r]);
+          const uint32_t mn = v[r] < o ? v[r] : o;
+          const uint32_t mx = v[r] < o ? o : v[r];
+          v[r] = lower ? mn : mx;
         }
       }
     }

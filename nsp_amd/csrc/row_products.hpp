@@ -8,8 +8,8 @@
 // in a window lane g handles products w + u*GS + g, u < R.  Windows are
 // software-pipelined: the B loads of window i+1 are issued before the
 // callback of window i runs, so loads stay in flight while the LDS hash work
-// of the previous window executes (two windows alternate roles, so nothing is
-// copied between them).  A product's entry is found without a search: a bit
+// of the previous window executes (for groups of 32 and 64 lanes the two
+// windows alternate roles, so nothing is copied between them).  A product's entry is found without a search: a bit
 // mask marks where entries start, one word per GS products, and
 //     entry = pre[word] + popc(word & lanes<=g) - 1
 // where pre[word] counts the entries starting before the word (entries with
@@ -153,14 +153,27 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
         for (int u = 0; u < R; u++) prod[u] = VAL ? es->av[W.e[u]] * W.bv[u] : (V)0;
         f(W.key, prod, W.ok, W.bv, W.e);
       };
-      Window<R, V> wa, wb;
-      load(0, wa);
-      for (int i = 0; i < nwin; i += 2) {
-        if (i + 1 < nwin) load(i + 1, wb);
-        consume(wa);
-        if (i + 1 < nwin) {
-          if (i + 2 < nwin) load(i + 2, wa);
-          consume(wb);
+      if constexpr (GS >= 32) {
+        // long rows: the two windows alternate roles (no copies)
+        Window<R, V> wa, wb;
+        load(0, wa);
+        for (int i = 0; i < nwin; i += 2) {
+          if (i + 1 < nwin) load(i + 1, wb);
+          consume(wa);
+          if (i + 1 < nwin) {
+            if (i + 2 < nwin) load(i + 2, wa);
+            consume(wb);
+          }
+        }
+      } else {
+        // short rows (mostly one window): fewer live registers, one more
+        // wave per SIMD in the small bins
+        Window<R, V> cur, nxt;
+        load(0, cur);
+        for (int i = 0; i < nwin; i++) {
+          if (i + 1 < nwin) load(i + 1, nxt);
+          consume(cur);
+          cur = nxt;
         }
       }
     }

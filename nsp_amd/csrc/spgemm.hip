@@ -362,8 +362,11 @@ __device__ __forceinline__ void clear_lds(T *p, int glane, int32_t fill) {
 // ---------------------------------------------------------------------------
 // Symbolic, GS lanes per row (sub-wave groups or one wave), SH-slot LDS key
 // table per row; products enumerated with R loads in flight per lane.
+// Groups of 8 and 16 lanes with tables of up to 256 slots have LDS for five
+// or more waves per SIMD; the register budget is held to that (96 VGPRs).
 template <int GS, int SH, int R>
-__global__ __launch_bounds__(256) void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GS <= 16 && SH <= 256 ? 5 : 1)))
+void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
                                                    const Info *__restrict__ info,
                                                    const int32_t *__restrict__ perm,
                                                    const int64_t *__restrict__ arpt,
