@@ -45,9 +45,11 @@ __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 // The multiply is written as the instruction itself: with only the low 24
 // bits of the product used, hipcc otherwise drops __umul24's operand mask and
 // emits the quarter-rate v_mul_lo_u32 (same low bits, four times the cycles).
+// (volatile: the compiler would otherwise sink it into the branches of its
+// users' selects instead of computing it unconditionally)
 __device__ __forceinline__ uint32_t hash_mul24(int32_t key) {
   uint32_t p;
-  asm("v_mul_u32_u24 %0, 0x9e3779, %1" : "=v"(p) : "v"(key));
+  asm volatile("v_mul_u32_u24 %0, 0x9e3779, %1" : "=v"(p) : "v"(key));
   return p & 0xFFFFFFu;
 }
 
@@ -95,8 +97,9 @@ __device__ __forceinline__ int hash_insert_keys(int32_t *keys, const int32_t (&k
   uint32_t h[R];
 #pragma unroll
   for (int u = 0; u < R; u++) {
+    const uint32_t hs = hash_slot<LOG>(key[u]);
     kk[u] = ok[u] ? key[u] : kDummy;
-    h[u] = ok[u] ? hash_slot<LOG>(key[u]) : SH + glane;
+    h[u] = ok[u] ? hs : SH + glane;
   }
 #pragma unroll
   for (int u = 0; u < R; u++) old[u] = atomicCAS(&keys[h[u]], kEmpty, kk[u]);
@@ -120,8 +123,9 @@ __device__ __forceinline__ void hash_accumulate_batch(int32_t *keys, V *vals, co
   uint32_t h[R];
 #pragma unroll
   for (int u = 0; u < R; u++) {
+    const uint32_t hs = hash_slot<LOG>(key[u]);
     kk[u] = ok[u] ? key[u] : kDummy;
-    h[u] = ok[u] ? hash_slot<LOG>(key[u]) : SH + glane;
+    h[u] = ok[u] ? hs : SH + glane;
   }
 #pragma unroll
   for (int u = 0; u < R; u++) old[u] = atomicCAS(&keys[h[u]], kEmpty, kk[u]);

@@ -176,6 +176,9 @@ __device__ __forceinline__ void finish_row(int64_t i, unsigned long long s, int6
   }
 }
 
+// Entries of a block's 256 rows staged through LDS per round of k_rowinfo.
+constexpr int kRowInfoStage = 4096;
+
 template <typename RowT>
 __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restrict__ arpt,
                                                  const int32_t *__restrict__ ab_len,
@@ -184,31 +187,72 @@ __global__ __launch_bounds__(256) void k_rowinfo(int64_t MA, const RowT *__restr
                                                  int32_t *__restrict__ long_rows, Info *info) {
   __shared__ unsigned long long hist[kBins];
   __shared__ unsigned long long s_nprod, s_max;
+  __shared__ int64_t s_e[2];
+  __shared__ int32_t stage[kRowInfoStage];
   if (threadIdx.x < kBins) hist[threadIdx.x] = 0;
   if (threadIdx.x == 0) { s_nprod = 0; s_max = 0; }
   __syncthreads();
   unsigned long long my_sum = 0, my_max = 0;
-  // one lane per row, its B row lengths loaded 8 at a time (independent
-  // loads in flight), so a wavefront has 64 rows in flight and the per-row
-  // outputs are written coalesced
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < MA; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p0 = (int64_t)arpt[i], p1 = (int64_t)arpt[i + 1];
-    if (p1 - p0 > kLongRow) {  // k_rowinfo_long finishes it
+  // One lane per row.  The block's 256 rows own one contiguous run of
+  // entries; while it spans at most four stages of kRowInfoStage entries it
+  // is read coalesced into LDS, stage by stage, and every lane sums its
+  // row's part of each stage there (rows one lane apart are ~row-length
+  // entries apart, so direct per-lane loads would touch a cache line each).
+  // Longer runs (a long row in the block) fall back to direct loads.
+  for (int64_t i0 = blockIdx.x * (int64_t)blockDim.x; i0 < MA; i0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = i0 + threadIdx.x;
+    const int64_t ilast = (i0 + (int64_t)blockDim.x < MA ? i0 + (int64_t)blockDim.x : MA) - 1;
+    const bool valid = i < MA;
+    int64_t p0 = 0, p1 = 0;
+    if (valid) {
+      p0 = (int64_t)arpt[i];
+      p1 = (int64_t)arpt[i + 1];
+    }
+    if (i == i0) s_e[0] = p0;
+    if (i == ilast) s_e[1] = p1;
+    const bool is_long = p1 - p0 > kLongRow;
+    if (is_long)  // k_rowinfo_long finishes it
       long_rows[atomicAdd((unsigned long long *)&info->nlong, 1ull)] = (int32_t)i;
-      continue;
-    }
-    int64_t s = 0, p = p0;
-    for (; p + 8 <= p1; p += 8) {
-      int32_t l[8];
+    const bool mine = valid && !is_long;
+    __syncthreads();
+    const int64_t e0 = s_e[0], e1 = s_e[1];
+    __syncthreads();  // s_e is rewritten by the next round
+    int64_t s = 0;
+    if (e1 - e0 <= 4 * (int64_t)kRowInfoStage) {
+      for (int64_t c0 = e0; c0 < e1; c0 += kRowInfoStage) {
+        const int64_t c1 = c0 + kRowInfoStage < e1 ? c0 + kRowInfoStage : e1;
+        constexpr int PER = kRowInfoStage / 256;
+        int32_t l[PER];
 #pragma unroll
-      for (int u = 0; u < 8; u++) l[u] = ab_len[p + u];
+        for (int u = 0; u < PER; u++) {
+          const int64_t p = c0 + u * 256 + threadIdx.x;
+          l[u] = p < c1 ? ab_len[p] : 0;
+        }
 #pragma unroll
-      for (int u = 0; u < 8; u++) s += l[u];
+        for (int u = 0; u < PER; u++) stage[u * 256 + threadIdx.x] = l[u];
+        __syncthreads();
+        if (mine) {
+          const int64_t a = p0 > c0 ? p0 : c0, b = p1 < c1 ? p1 : c1;
+          for (int64_t p = a; p < b; p++) s += stage[p - c0];
+        }
+        __syncthreads();
+      }
+    } else if (mine) {
+      int64_t p = p0;
+      for (; p + 8 <= p1; p += 8) {
+        int32_t l[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) l[u] = ab_len[p + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += l[u];
+      }
+      for (; p < p1; p++) s += ab_len[p];
     }
-    for (; p < p1; p++) s += ab_len[p];
-    finish_row(i, (unsigned long long)s, p0, intprod, arpt64, row_nz, cls, hist);
-    my_sum += (unsigned long long)s;
-    my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
+    if (mine) {
+      finish_row(i, (unsigned long long)s, p0, intprod, arpt64, row_nz, cls, hist);
+      my_sum += (unsigned long long)s;
+      my_max = my_max > (unsigned long long)s ? my_max : (unsigned long long)s;
+    }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) arpt64[MA] = (int64_t)arpt[MA];
   my_sum = group_sum<64>(my_sum);

@@ -276,3 +276,27 @@ def test_small_output_rows_with_many_products(nsp, orc, rpw, dtype):
     st = h.stats()
     assert st["num_bin"][2] > 0 and sum(st["num_bin"][3:]) == 0, st["num_bin"]
     check_against_oracle(A, B, C, dtype, orc)
+
+
+@pytest.mark.parametrize("long_len", [1500, 12000])
+def test_intprod_blocks_with_long_rows(nsp, orc, long_len):
+    """Intermediate-product counts where 256-row blocks hold a long row
+    (more than 1024 entries, summed by the long-row kernel): with 1500
+    entries the block's entry run (8.7 K) still fits the four LDS stages
+    of 4096, with 12000 (19.2 K) it does not and the block reads its rows directly; the
+    other blocks span one to four stages."""
+    print("case: intprod with long rows", long_len, flush=True)
+    rng = np.random.default_rng(41)
+    nA, nB = 2048, 16000
+    lens = rng.integers(0, 60, nA)
+    lens[300] = long_len
+    lens[1800] = 1025
+    rows = np.repeat(np.arange(nA), lens)
+    cols = np.concatenate([rng.choice(nB, int(k), replace=False) for k in lens])
+    A = gen._from_rows_cols(nA, nB, rows, cols, rng.uniform(-1, 1, rows.size))
+    B = gen.random_csr(nB, 500, 0.02, seed=42, empty_rows=0.1)
+    h = nsp.Handle()
+    ip, total = h.intprod(nsp.DeviceCSR.from_host(A), nsp.DeviceCSR.from_host(B))
+    ref, rtotal = orc.intprod(A, B)
+    np.testing.assert_array_equal(ip.cpu().numpy(), ref)
+    assert total == rtotal
