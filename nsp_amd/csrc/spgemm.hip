@@ -1199,11 +1199,14 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
   const int64_t nnzA = A->nnz;
   if (nnzA > 0) {
     const int ge = (int)std::max<int64_t>(
-        1, std::min<int64_t>((nnzA + 256 * kExtU - 1) / (256 * kExtU), (int64_t)h->num_cu * 16));
+        1, std::min<int64_t>((nnzA + 256 * kExtU - 1) / (256 * kExtU), (int64_t)h->num_cu * 8));
     hipLaunchKernelGGL(k_extents<RowT>, dim3(ge), dim3(256), 0, h->stream, nnzA, (const RowT *)A->row_ptr, A->col,
                        (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->ab_b0, h->ab_len);
   }
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 255) / 256, (int64_t)h->num_cu * 8));
+  // two blocks per CU: every block ends with a few global atomics on the
+  // same counters, which serialise (8 blocks per CU: 58 us on the 524 K-row
+  // stencil share, 2 per CU: 33 us)
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 255) / 256, (int64_t)h->num_cu * 2));
   // perm_num is free until the numeric binning: it holds the long-row list
   hipLaunchKernelGGL(k_rowinfo<RowT>, dim3(g), dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr, h->ab_len,
                      h->a_rpt64, h->intprod, h->row_nz, h->cls, h->perm_num, h->d_info);
