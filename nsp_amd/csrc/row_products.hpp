@@ -56,12 +56,14 @@ struct Window {
 // entries, entry[u] is the product's compacted entry index in the chunk.
 // VAL = true loads B values and forms a_ik*b_kj; LOADB = true loads B values
 // without A values (products are then 0); both false skips values (symbolic
-// pass).  Within a group all GS lanes must be active; different groups of a
+// pass).  O32 = true (B with at most 2^29 entries) addresses B's entries
+// with 32-bit byte offsets from the arrays' bases (saddr loads, no 64-bit
+// address arithmetic per product).  Within a group all GS lanes must be active; different groups of a
 // wave may be in different iterations.  Entries are taken from chunks
 // [c0, c0+GS), c0 = a0 + first, first + step, ... (step = GS for a group that
 // owns the row; a block splits a row's chunks over its waves with
 // first = wave*64, step = 64*waves).
-template <int GS, int R, bool VAL, typename V, bool LOADB = VAL, typename F>
+template <int GS, int R, bool VAL, typename V, bool LOADB = VAL, bool O32 = false, typename F>
 __device__ __forceinline__ void row_products(const int glane, const int64_t a0, const int64_t a1,
                                              const int64_t first, const int64_t step,
                                              const int64_t *__restrict__ ab_b0,
@@ -141,9 +143,17 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
             e = (int)(m >> 32) + __popc((uint32_t)m & (uint32_t)le) - 1;
           W.ok[u] = q * GS + glane < tend;
           e = W.ok[u] ? e : 0;
-          const int64_t jb = es->base[e] + (W.ok[u] ? sw + q * GS + glane : 0);
-          W.key[u] = bcol[jb];
-          if (VAL || LOADB) W.bv[u] = bval[jb];
+          if constexpr (O32) {
+            const uint32_t jb = (uint32_t)es->base[e] + (uint32_t)(W.ok[u] ? sw + q * GS + glane : 0);
+            W.key[u] = *reinterpret_cast<const int32_t *>(reinterpret_cast<const char *>(bcol) + (uint64_t)(jb * 4u));
+            if (VAL || LOADB)
+              W.bv[u] = *reinterpret_cast<const V *>(reinterpret_cast<const char *>(bval) +
+                                                     (uint64_t)(jb * (uint32_t)sizeof(V)));
+          } else {
+            const int64_t jb = es->base[e] + (W.ok[u] ? sw + q * GS + glane : 0);
+            W.key[u] = bcol[jb];
+            if (VAL || LOADB) W.bv[u] = bval[jb];
+          }
           W.e[u] = e;
         }
       };

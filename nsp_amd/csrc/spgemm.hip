@@ -408,7 +408,7 @@ __device__ __forceinline__ void clear_lds(T *p, int glane, int32_t fill) {
 // table per row; products enumerated with R loads in flight per lane.
 // Groups of 8 and 16 lanes with tables of up to 256 slots have LDS for five
 // or more waves per SIMD; the register budget is held to that (96 VGPRs).
-template <int GS, int SH, int R>
+template <int GS, int SH, int R, bool O32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GS <= 16 && SH <= 256 ? 5 : 1)))
 void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
                                                    const Info *__restrict__ info,
@@ -434,7 +434,7 @@ void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
       clear_lds<GS, SHD>(K, glane, kEmpty);
       wave_lds_sync();
       int cnt = 0;
-      row_products<GS, R, false, float>(
+      row_products<GS, R, false, float, false, O32>(
           glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[slot],
           [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
             cnt += hash_insert_keys<LOG, R>(K, key, ok, glane);
@@ -566,7 +566,7 @@ using Acc = double;
 // Numeric, GS lanes per row, SH-slot LDS table (keys + values) per row, then
 // compaction of the occupied slots, a register bitonic sort of the row's
 // columns (GS lanes x E keys) and a lookup of each sorted column's value.
-template <typename V, int GS, int NS, int SHF, int R>
+template <typename V, int GS, int NS, int SHF, int R, bool O32>
 __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64_t direct_rows,
                                                    const Info *__restrict__ info,
                                                    const int32_t *__restrict__ perm,
@@ -607,7 +607,7 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
       clear_lds<GS, SHD>(K, glane, kEmpty);
       clear_lds<GS, SHD>(W, glane, 0);
       wave_lds_sync();
-      row_products<GS, R, true, V>(glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, aval, bcol, bval,
+      row_products<GS, R, true, V, true, O32>(glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, aval, bcol, bval,
                                    &ent[slot],
                                    [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
                                      Acc pa[R];
@@ -961,17 +961,26 @@ constexpr int kNumR = 4;  // B loads in flight per lane, numeric
 
 // Launches the kernel of bin `bin` over bins [lo, bin] (rows of lower bins fit
 // its larger table), or over all rows in natural order when direct >= 0.
+// Whether B's entries can be addressed with 32-bit byte offsets (8-byte
+// values: at most 2^29 entries).
+static bool offsets32(int64_t nnzB) { return nnzB <= ((int64_t)1 << 29); }
+
 static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
                            const int64_t *arpt, const int64_t *ab_b0, const int32_t *ab_len,
-                           const int32_t *bcol) {
+                           const int32_t *bcol, int64_t nnzB) {
   hipStream_t st = h->stream;
   const Info *info = h->d_info;
   const int32_t *perm = direct >= 0 ? nullptr : h->perm_sym;
   int32_t *rnz = h->row_nz;
   const int cap = h->grid_cap;
+  const bool o32 = offsets32(nnzB);
 #define SYM_GROUP(GS, SH)                                                                          \
-  hipLaunchKernelGGL((k_sym_group<GS, SH, kSymR>), dim3(grid_for(h, count, 256 / GS, cap)),        \
-                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
+  if (o32)                                                                                         \
+    hipLaunchKernelGGL((k_sym_group<GS, SH, kSymR, true>), dim3(grid_for(h, count, 256 / GS, cap)), \
+                       dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz); \
+  else                                                                                             \
+    hipLaunchKernelGGL((k_sym_group<GS, SH, kSymR, false>), dim3(grid_for(h, count, 256 / GS, cap)), \
+                       dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
 #define SYM_BLOCK(BS, SH)                                                                          \
   hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, cap)), dim3(BS), 0, \
                      st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
@@ -1017,16 +1026,22 @@ template <typename V>
 static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
                            const int64_t *arpt, const int64_t *ab_b0, const V *aval,
                            const int32_t *ab_len, const int32_t *bcol, const V *bval,
-                           int32_t *ccol, V *cval, int64_t ncols) {
+                           int32_t *ccol, V *cval, int64_t ncols, int64_t nnzB) {
   hipStream_t st = h->stream;
   const Info *info = h->d_info;
   const int32_t *perm = direct >= 0 ? nullptr : h->perm_num;
   const int64_t *crpt = h->c_rpt64;
   const int cap = h->grid_cap;
+  const bool o32 = offsets32(nnzB);
 #define NUM_GROUP(GS, NS, SHF)                                                                      \
-  hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR>), dim3(grid_for(h, count, 256 / GS, cap)),      \
-                     dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,      \
-                     bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF))
+  if (o32)                                                                                          \
+    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR, true>), dim3(grid_for(h, count, 256 / GS, cap)), \
+                       dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,    \
+                       bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF));                     \
+  else                                                                                              \
+    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR, false>), dim3(grid_for(h, count, 256 / GS, cap)), \
+                       dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,    \
+                       bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF))
 #define NUM_BLOCK(BS, NS, SHF)                                                                      \
   hipLaunchKernelGGL((k_num_block<V, BS, NS, SHF, kNumR>), dim3(grid_for(h, count, 1, cap)), dim3(BS),   \
                      0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval,     \
@@ -1242,7 +1257,8 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
       launch_scatter(h, MA, h->d_info->count_sym, h->d_info->cursor_sym, h->perm_sym);
     timing_mark(h, 1);
     for (int q = 0; q < n; q++)
-      launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, h->ab_len, B->col);
+      launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, h->ab_len, B->col,
+                     B->nnz);
     h->stats.sym_launches = n;
     if (n == 1 && L[0].direct >= 0) h->stats.direct_mode |= 1;
   }
@@ -1307,7 +1323,7 @@ static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
   timing_mark(h, 4);
   for (int q = 0; q < n; q++)
     launch_num_bin<V>(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, (const V *)A->val,
-                      h->ab_len, B->col, (const V *)B->val, C->col, (V *)C->val, B->cols);
+                      h->ab_len, B->col, (const V *)B->val, C->col, (V *)C->val, B->cols, B->nnz);
   h->stats.num_launches = n;
   h->stats.direct_mode = (h->stats.direct_mode & 1) | ((n == 1 && L[0].direct >= 0) ? 2 : 0);
   timing_mark(h, 5);
@@ -1377,13 +1393,13 @@ static int graph_record(Handle *g, const nsp_csr_t *A, const nsp_csr_t *B, const
   NSP_TRY(run_rowinfo<RowT>(g, A, B, bt, C->nnz));
   launch_scatter(g, MA, g->d_info->count_sym, g->d_info->cursor_sym, g->perm_sym);
   for (int b = 0; b < kBins; b++)
-    launch_sym_bin(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, g->ab_len, B->col);
+    launch_sym_bin(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, g->ab_len, B->col, B->nnz);
   hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)(MA / kScanTile + 1)), dim3(kScanBS), 0, g->stream, MA, g->row_nz,
                      (RowT *)C->row_ptr, g->c_rpt64, g->cls, g->tile_status, g->d_info);
   launch_scatter(g, MA, g->d_info->count_num, g->d_info->cursor_num, g->perm_num);
   for (int b = 0; b < kBins; b++)
     launch_num_bin<V>(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len,
-                      B->col, (const V *)B->val, C->col, (V *)C->val, B->cols);
+                      B->col, (const V *)B->val, C->col, (V *)C->val, B->cols, B->nnz);
   NSP_HIP(g, hipGetLastError());
   return NSP_OK;
 }

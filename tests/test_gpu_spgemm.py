@@ -300,3 +300,46 @@ def test_intprod_blocks_with_long_rows(nsp, orc, long_len):
     ref, rtotal = orc.intprod(A, B)
     np.testing.assert_array_equal(ip.cpu().numpy(), ref)
     assert total == rtotal
+
+
+def test_b_beyond_32bit_offsets(nsp):
+    """B with 2^20 rows of 513 entries (537.9 M > 2^29 entries, FP32): the
+    group kernels address B with 64-bit offsets (32-bit byte offsets serve
+    only B with at most 2^29 entries).  A's 64 rows take B rows near the end
+    of B, whose entries sit past 2^29; C is checked against a host
+    accumulation of the same products (FP32 products summed in FP64 and
+    rounded once, as the kernels do)."""
+    import torch
+    print("case: B beyond 32-bit offsets (FP32, 537.9 M entries)", flush=True)
+    nB, per, ncols = 1 << 20, 513, 1 << 21
+    r = np.arange(nB, dtype=np.int64)
+    bcol = ((r * 7)[:, None] + np.arange(per, dtype=np.int64)[None, :]) % ncols
+    bcol.sort(axis=1)
+    bcol = bcol.astype(np.int32).ravel()
+    bval = (1.0 + (bcol % 3)).astype(np.float32)
+    brpt = np.arange(nB + 1, dtype=np.int64) * per
+    assert brpt[-1] > (1 << 29)
+    rng = np.random.default_rng(51)
+    nA, k = 64, 8
+    acol = np.sort(np.stack([nB - 1 - rng.choice(4096, k, replace=False) for _ in range(nA)]), axis=1)
+    aval = rng.uniform(-1, 1, (nA, k)).astype(np.float32)
+    arpt = np.arange(nA + 1, dtype=np.int64) * k
+    dev = torch.device("cuda")
+    dA = nsp.DeviceCSR(torch.from_numpy(arpt.astype(np.int32)).to(dev), torch.from_numpy(acol.astype(np.int32).ravel()).to(dev),
+                       torch.from_numpy(aval.ravel()).to(dev), nA, nB)
+    dB = nsp.DeviceCSR(torch.from_numpy(brpt.astype(np.int32)).to(dev), torch.from_numpy(bcol).to(dev),
+                       torch.from_numpy(bval).to(dev), nB, ncols)
+    C = nsp.Handle().spgemm(dA, dB)
+    crpt = C.row_ptr.cpu().numpy().astype(np.int64)
+    ccol, cval = C.col.cpu().numpy(), C.val.cpu().numpy()
+    del dA, dB, C
+    torch.cuda.empty_cache()
+    for i in range(nA):
+        cols = np.concatenate([bcol[brpt[j]:brpt[j + 1]] for j in acol[i]])
+        prods = np.concatenate([np.float64(np.float32(aval[i, q]) * bval[brpt[j]:brpt[j + 1]])
+                                for q, j in enumerate(acol[i])])
+        uc, inv = np.unique(cols, return_inverse=True)
+        acc = np.zeros(uc.size)
+        np.add.at(acc, inv, prods)
+        np.testing.assert_array_equal(ccol[crpt[i]:crpt[i + 1]], uc)
+        np.testing.assert_allclose(cval[crpt[i]:crpt[i + 1]], acc.astype(np.float32), rtol=1e-6, atol=1e-6)
