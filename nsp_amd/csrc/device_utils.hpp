@@ -134,8 +134,8 @@ __device__ __forceinline__ void hash_accumulate_batch(int32_t *keys, V *vals, co
     int ins = 0;
     if (old[u] != kEmpty && old[u] != kk[u]) h[u] = hash_probe_from<LOG>(keys, kk[u], h[u], ins);
   }
-#pragma unroll
   // lanes with ok[u] false add into their dummy slot, which no reader visits
+#pragma unroll
   for (int u = 0; u < R; u++)
     __hip_atomic_fetch_add(&vals[h[u]], prod[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }

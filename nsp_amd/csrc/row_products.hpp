@@ -9,8 +9,9 @@
 // software-pipelined: the B loads of window i+1 are issued before the
 // callback of window i runs, so loads stay in flight while the LDS hash work
 // of the previous window executes (for groups of 32 and 64 lanes the two
-// windows alternate roles, so nothing is copied between them).  A product's entry is found without a search: a bit
-// mask marks where entries start, one word per GS products, and
+// windows alternate roles, so nothing is copied between them).  A product's
+// entry is found without a search: a bit mask marks where entries start, one
+// word per GS products, and
 //     entry = pre[word] + popc(word & lanes<=g) - 1
 // where pre[word] counts the entries starting before the word (entries with
 // empty B rows are compacted away first).  Masks and counts cover a
@@ -58,8 +59,8 @@ struct Window {
 // without A values (products are then 0); both false skips values (symbolic
 // pass).  O32 = true (B with at most 2^29 entries) addresses B's entries
 // with 32-bit byte offsets from the arrays' bases (saddr loads, no 64-bit
-// address arithmetic per product).  Within a group all GS lanes must be active; different groups of a
-// wave may be in different iterations.  Entries are taken from chunks
+// address arithmetic per product).  Within a group all GS lanes must be
+// active; different groups of a wave may be in different iterations.  Entries are taken from chunks
 // [c0, c0+GS), c0 = a0 + first, first + step, ... (step = GS for a group that
 // owns the row; a block splits a row's chunks over its waves with
 // first = wave*64, step = 64*waves).
