@@ -303,33 +303,44 @@ def test_intprod_blocks_with_long_rows(nsp, orc, long_len):
 
 
 def test_b_beyond_32bit_offsets(nsp):
-    """B with 2^20 rows of 513 entries (537.9 M > 2^29 entries, FP32): the
-    group kernels address B with 64-bit offsets (32-bit byte offsets serve
-    only B with at most 2^29 entries).  A's 64 rows take B rows near the end
-    of B, whose entries sit past 2^29; C is checked against a host
-    accumulation of the same products (FP32 products summed in FP64 and
-    rounded once, as the kernels do)."""
+    """B with 537.9 M > 2^29 entries (FP32): 2^20 - 64 rows of 513 entries,
+    then 64 rows of 32.  A's 64 rows take 3 of those last rows each (96
+    products, at most 96 entries: the wave-level kernels), whose entries sit
+    past 2^29, so the group kernels address B with 64-bit offsets (32-bit
+    byte offsets serve only B with at most 2^29 entries).  C is checked
+    against a host accumulation of the same products (FP32 products summed in
+    FP64 and rounded once, as the kernels do)."""
     import torch
     print("case: B beyond 32-bit offsets (FP32, 537.9 M entries)", flush=True)
-    nB, per, ncols = 1 << 20, 513, 1 << 21
-    r = np.arange(nB, dtype=np.int64)
-    bcol = ((r * 7)[:, None] + np.arange(per, dtype=np.int64)[None, :]) % ncols
-    bcol.sort(axis=1)
-    bcol = bcol.astype(np.int32).ravel()
+    nB, per, short, ncols = 1 << 20, 513, 32, 1 << 21
+    nlong = nB - 64
+    r = np.arange(nlong, dtype=np.int64)
+    blong = ((r * 7)[:, None] + np.arange(per, dtype=np.int64)[None, :]) % ncols
+    blong.sort(axis=1)
+    rs = np.arange(nlong, nB, dtype=np.int64)
+    bshort = ((rs * 7)[:, None] + np.arange(short, dtype=np.int64)[None, :]) % ncols
+    bshort.sort(axis=1)
+    bcol = np.concatenate([blong.astype(np.int32).ravel(), bshort.astype(np.int32).ravel()])
+    del blong
     bval = (1.0 + (bcol % 3)).astype(np.float32)
-    brpt = np.arange(nB + 1, dtype=np.int64) * per
-    assert brpt[-1] > (1 << 29)
+    brpt = np.concatenate([np.arange(nlong + 1, dtype=np.int64) * per,
+                           nlong * per + np.arange(1, 65, dtype=np.int64) * short])
+    assert brpt[-1] == bcol.size and bcol.size > (1 << 29) and brpt[nlong] > (1 << 29)
     rng = np.random.default_rng(51)
-    nA, k = 64, 8
-    acol = np.sort(np.stack([nB - 1 - rng.choice(4096, k, replace=False) for _ in range(nA)]), axis=1)
+    nA, k = 64, 3
+    acol = np.sort(np.stack([nlong + rng.choice(64, k, replace=False) for _ in range(nA)]), axis=1)
     aval = rng.uniform(-1, 1, (nA, k)).astype(np.float32)
     arpt = np.arange(nA + 1, dtype=np.int64) * k
     dev = torch.device("cuda")
-    dA = nsp.DeviceCSR(torch.from_numpy(arpt.astype(np.int32)).to(dev), torch.from_numpy(acol.astype(np.int32).ravel()).to(dev),
-                       torch.from_numpy(aval.ravel()).to(dev), nA, nB)
+    dA = nsp.DeviceCSR(torch.from_numpy(arpt.astype(np.int32)).to(dev),
+                       torch.from_numpy(acol.astype(np.int32).ravel()).to(dev), torch.from_numpy(aval.ravel()).to(dev),
+                       nA, nB)
     dB = nsp.DeviceCSR(torch.from_numpy(brpt.astype(np.int32)).to(dev), torch.from_numpy(bcol).to(dev),
                        torch.from_numpy(bval).to(dev), nB, ncols)
-    C = nsp.Handle().spgemm(dA, dB)
+    h = nsp.Handle()
+    C = h.spgemm(dA, dB)
+    st = h.stats()
+    assert sum(st["sym_bin"][6:]) == 0 and sum(st["num_bin"][6:]) == 0, (st["sym_bin"], st["num_bin"])
     crpt = C.row_ptr.cpu().numpy().astype(np.int64)
     ccol, cval = C.col.cpu().numpy(), C.val.cpu().numpy()
     del dA, dB, C
