@@ -354,3 +354,39 @@ def test_b_beyond_32bit_offsets(nsp):
         np.add.at(acc, inv, prods)
         np.testing.assert_array_equal(ccol[crpt[i]:crpt[i + 1]], uc)
         np.testing.assert_allclose(cval[crpt[i]:crpt[i + 1]], acc.astype(np.float32), rtol=1e-6, atol=1e-6)
+
+
+def test_allocation_failure_returns_oom(nsp, orc):
+    """nsp_spgemm with an allocator that fails on its second request (C's
+    columns): NSP_ERR_OOM, C zeroed, every buffer it did get handed back to
+    the free callback, and the handle still computes the next product."""
+    import ctypes as C
+    import torch
+    from nsp_amd import _lib
+    print("case: allocation failure", flush=True)
+    A = gen.random_csr(200, 150, 0.05, seed=71)
+    B = gen.random_csr(150, 120, 0.05, seed=72)
+    dA, dB = nsp.DeviceCSR.from_host(A), nsp.DeviceCSR.from_host(B)
+    keep, calls = {}, [0]
+
+    def _alloc(nbytes, user):
+        calls[0] += 1
+        if calls[0] == 2:
+            return None
+        t = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device="cuda")
+        keep[t.data_ptr()] = t
+        return t.data_ptr()
+
+    def _free(ptr, user):
+        if ptr:
+            keep.pop(ptr)
+
+    alloc, free = _lib.ALLOC_FN(_alloc), _lib.FREE_FN(_free)
+    h = nsp.Handle()
+    a, b, c = dA.c_struct(), dB.c_struct(), _lib.nsp_csr_t()
+    status = h.L.nsp_spgemm(h.h, dA.index_type, dA.value_type, C.byref(a), C.byref(b), C.byref(c),
+                            C.cast(alloc, C.c_void_p), C.cast(free, C.c_void_p), None)
+    assert status == _lib.NSP_ERR_OOM
+    assert calls[0] == 3 and not keep, (calls, keep)
+    assert not c.row_ptr and not c.col and not c.val and c.nnz == 0
+    check_against_oracle(A, B, h.spgemm(dA, dB), "f64", orc)
