@@ -406,8 +406,8 @@ def main():
                     help="library tuning parameter name=value (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(the multi-GPU sharding is described in DESIGN.md)"}))
+        print(json.dumps({"error": "the local product runs on one GPU; --gpus above 1 is not supported "
+                                   "(rows of A would shard one block per GPU, with no exchange inside the product)"}))
         sys.exit(2)
 
     import torch
