@@ -245,6 +245,33 @@ def test_local_share_stitches_to_full_product(orc):
             np.testing.assert_allclose(lval[seg], cval[ref], rtol=1e-14, atol=1e-15)
 
 
+@pytest.mark.parametrize("r0,r1", [(0, 30), (30, 60), (60, 90)])
+def test_completed_rows_and_gather_closed_form(orc, r0, r1):
+    """completedP's rows for a first, middle and last block, against their
+    definition (src/spspmpi.cu:37-55, 395-906): the global rows
+    sorted(unique((cols + r0) U [r0, r0 + l])), which puts the halo rows below
+    the block, then the block, then the halo rows above, and each A column
+    re-expressed as its position in that list; each gathered row is B's row
+    with columns moved to the rank frame (global - r0, spspmpi.cu:822-828)."""
+    A = gen.random_csr(90, 90, 0.06, seed=33, empty_rows=0.05)
+    A_rank = A.row_slice(r0, r1, col_shift=r0)
+    l = (r1 - r0) - 1
+    grows, newcol = orc.completed_rows(A_rank.col, l, r0)
+    gcols = A_rank.col.astype(np.int64) + r0
+    expect = np.unique(np.concatenate([gcols, np.arange(r0, r0 + l + 1, dtype=np.int64)]))
+    np.testing.assert_array_equal(grows, expect)
+    np.testing.assert_array_equal(grows[newcol], gcols)
+    below, local, above = grows[grows < r0], grows[(grows >= r0) & (grows < r1)], grows[grows >= r1]
+    np.testing.assert_array_equal(local, np.arange(r0, r1))
+    np.testing.assert_array_equal(grows, np.concatenate([below, local, above]))
+    prpt, pcol, pval = orc.gather_rows(A, grows, r0)
+    assert prpt[0] == 0 and prpt.size == grows.size + 1
+    for i, g in enumerate(grows):
+        b, e = A.rpt[g], A.rpt[g + 1]
+        np.testing.assert_array_equal(pcol[prpt[i]:prpt[i + 1]], A.col[b:e].astype(np.int64) - r0)
+        np.testing.assert_array_equal(pval[prpt[i]:prpt[i + 1]], A.val[b:e])
+
+
 def test_config2_share_is_identity_map(orc):
     """Rank 0 of 4 of the 27-pt stencil: both compaction maps are the
     identity (m=16 keeps it small; 128^3 gives shrinked_m 540,672/557,056)."""
