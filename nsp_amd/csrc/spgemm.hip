@@ -956,8 +956,18 @@ static int heavy_grid(const Handle *h, int64_t rows) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(rows, cap));
 }
 
-constexpr int kSymR = 8;  // B loads in flight per lane, symbolic
-constexpr int kNumR = 4;  // B loads in flight per lane, numeric
+constexpr int kSymR = 8;  // B loads in flight per lane, symbolic (at most)
+
+// Loads per lane of a symbolic group kernel: a window of GS*R products never
+// exceeds the bin's product bound SH (tables hold SH >= the row's products),
+// so the lanes of a short row do not issue dummy inserts; at least 2.
+constexpr int sym_r(int GS, int SH) { return SH / GS >= kSymR ? kSymR : (SH / GS >= 4 ? 4 : 2); }
+constexpr int kNumR = 4;  // B loads in flight per lane, numeric (at most)
+
+// Loads per lane of a numeric group kernel: a window of GS*R products is
+// held near twice the bin's row-length bound NS (a row has at least as many
+// products as entries, typically a few times more); at least 2.
+constexpr int num_r(int GS, int NS) { return 2 * NS / GS >= kNumR ? kNumR : 2; }
 
 // Launches the kernel of bin `bin` over bins [lo, bin] (rows of lower bins fit
 // its larger table), or over all rows in natural order when direct >= 0.
@@ -976,10 +986,10 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
   const bool o32 = offsets32(nnzB);
 #define SYM_GROUP(GS, SH)                                                                          \
   if (o32)                                                                                         \
-    hipLaunchKernelGGL((k_sym_group<GS, SH, kSymR, true>), dim3(grid_for(h, count, 256 / GS, cap)), \
+    hipLaunchKernelGGL((k_sym_group<GS, SH, sym_r(GS, SH), true>), dim3(grid_for(h, count, 256 / GS, cap)), \
                        dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz); \
   else                                                                                             \
-    hipLaunchKernelGGL((k_sym_group<GS, SH, kSymR, false>), dim3(grid_for(h, count, 256 / GS, cap)), \
+    hipLaunchKernelGGL((k_sym_group<GS, SH, sym_r(GS, SH), false>), dim3(grid_for(h, count, 256 / GS, cap)), \
                        dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
 #define SYM_BLOCK(BS, SH)                                                                          \
   hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, cap)), dim3(BS), 0, \
@@ -1035,11 +1045,11 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
   const bool o32 = offsets32(nnzB);
 #define NUM_GROUP(GS, NS, SHF)                                                                      \
   if (o32)                                                                                          \
-    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR, true>), dim3(grid_for(h, count, 256 / GS, cap)), \
+    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, num_r(GS, NS), true>), dim3(grid_for(h, count, 256 / GS, cap)), \
                        dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,    \
                        bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF));                     \
   else                                                                                              \
-    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, kNumR, false>), dim3(grid_for(h, count, 256 / GS, cap)), \
+    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, num_r(GS, NS), false>), dim3(grid_for(h, count, 256 / GS, cap)), \
                        dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,    \
                        bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF))
 #define NUM_BLOCK(BS, NS, SHF)                                                                      \
