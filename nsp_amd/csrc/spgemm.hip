@@ -965,9 +965,9 @@ constexpr int sym_r(int GS, int SH) { return SH / GS >= kSymR ? kSymR : (SH / GS
 constexpr int kNumR = 4;  // B loads in flight per lane, numeric (at most)
 
 // Loads per lane of a numeric group kernel: a window of GS*R products is
-// held near twice the bin's row-length bound NS (a row has at least as many
-// products as entries, typically a few times more); at least 2.
-constexpr int num_r(int GS, int NS) { return 2 * NS / GS >= kNumR ? kNumR : 2; }
+// held near the bin's row-length bound NS (a row has at least as many
+// products as entries); at least 2.
+constexpr int num_r(int GS, int NS) { return NS / GS >= kNumR ? kNumR : 2; }
 
 // Launches the kernel of bin `bin` over bins [lo, bin] (rows of lower bins fit
 // its larger table), or over all rows in natural order when direct >= 0.
