@@ -127,8 +127,13 @@ __device__ __forceinline__ void hash_accumulate_batch(int32_t *keys, V *vals, co
     kk[u] = ok[u] ? key[u] : kDummy;
     h[u] = ok[u] ? hs : SH + glane;
   }
+  // a product slot that no lane of the wavefront fills (the tail of a row's
+  // last window) issues neither the swap nor the add
+  bool live[R];
 #pragma unroll
-  for (int u = 0; u < R; u++) old[u] = atomicCAS(&keys[h[u]], kEmpty, kk[u]);
+  for (int u = 0; u < R; u++) live[u] = __ballot(ok[u]) != 0;
+#pragma unroll
+  for (int u = 0; u < R; u++) old[u] = live[u] ? atomicCAS(&keys[h[u]], kEmpty, kk[u]) : kk[u];
 #pragma unroll
   for (int u = 0; u < R; u++) {
     int ins = 0;
@@ -137,7 +142,7 @@ __device__ __forceinline__ void hash_accumulate_batch(int32_t *keys, V *vals, co
   // lanes with ok[u] false add into their dummy slot, which no reader visits
 #pragma unroll
   for (int u = 0; u < R; u++)
-    __hip_atomic_fetch_add(&vals[h[u]], prod[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (live[u]) __hip_atomic_fetch_add(&vals[h[u]], prod[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---- group (sub-wave) collectives; GS lanes, GS a power of two <= 64 ------
