@@ -160,7 +160,7 @@ CSR *galerkin_product(CSR *R, CSR *A, CSR *P);
  * first to the last row present in the rank's range, columns shifted by
  * row_shift = the rows of the ranks before (the reference's MPI_Exscan of
  * rowxproc; the block's first row unless empty rows sit at partition ends),
- * rows sorted by column.  *nr_out/*nc_out/*nt_out get the file's sizes. */
+ * rows sorted by column.  *nr_out, *nc_out and *nt_out get the file's sizes. */
 CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type = 1, int64_t *nr_out = nullptr,
                         int64_t *nc_out = nullptr, int64_t *nt_out = nullptr);
 /* The same block as a device CSR built on the device (nsp_coo_read: records
