@@ -210,8 +210,11 @@ int nsp_transpose(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_
  * AT is A's transpose from nsp_transpose.  at_perm (optional, index type
  * `it`, AT->nnz entries) maps each A^T entry to its A entry; when it is given
  * A's values are read as A->val[at_perm[t]] and AT->val may be NULL, so a
- * cached A^T pattern serves any values of A.  Deterministic.  No reference
- * counterpart (the reference has no backward); C rows of up to 1024 entries. */
+ * cached A^T pattern serves any values of A.  Deterministic.  C rows of any
+ * length: up to 1024 entries through an LDS table of the row, longer ones by
+ * binary search in the sorted C row.  No host synchronisation when the
+ * handle's cached B row extents apply.  No reference counterpart (the
+ * reference has no backward). */
 int nsp_spgemm_backward(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A,
                         const nsp_csr_t *B, const nsp_csr_t *AT, const void *at_perm,
                         const nsp_csr_t *C, const void *dC, void *dA, void *dB);

@@ -4,7 +4,9 @@
  * Names, fields and argument meaning follow the reference (amavliut/nsp):
  *   sfCSR, nsp_spgemm_kernel_hash     nsp/inc/nsp.h:57-70, 105
  *   csrlocinfo                        src/csrseg.h:3-9
- *   CSR (fields the product uses)     src/CSR.h:67-104
+ *   stype, gstype, gsstype, itype     src/setting.h:21-28
+ *   CSR (CSRtype<>)                   src/CSR.h:67-104
+ *   handles, Handles::init/free       src/handles.h:5-16
  *   CSRm::init/free/copyToDevice/...  src/CSR.cu:28, 572-621, 656-700
  *   shrink_col, get_shrinked_matrix   src/spspmpi.cu:106-182
  *   nsparseMGPU_version0              src/spspmpi.cu:266-313
@@ -22,8 +24,11 @@
  *   galerkin_product                  R(AP), named at main.cu:138
  * Errors: where the reference exits (checkCudaErrors, helper_cuda.h:714) these
  * functions throw nspamd::Error carrying the C ABI status.  Calls are
- * host-synchronous on return, as in the reference; each host thread uses its
- * own library handle on the current device.
+ * host-synchronous on return and ordered on the legacy (null) stream, as in
+ * the reference; each host thread uses its own library handle on the current
+ * device.  Code written against the reference compiles against this header
+ * with `using namespace nspamd;` (CSR, CSRm::, Handles::, stype ... resolve
+ * to the declarations below).
  *
  * Link with -lnsp_amd_cpp -lnsp_amd (and the HIP runtime). */
 #ifndef NSP_AMD_HPP
@@ -33,6 +38,8 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+
+#include <hip/hip_runtime.h>
 
 #include "nsp_amd.h"
 
@@ -75,37 +82,58 @@ typedef struct {
   double *val;
 } csrlocinfo;
 
-/* The CSR fields of src/CSR.h:67-104 that the local product and its
- * compaction read or write.  Arrays are device pointers when on_the_device,
- * host pointers otherwise.  Columns of a rank's block are in the rank frame
- * (global column - row_shift). */
+/* The reference's index and size types (src/setting.h:21-28): matrix sizes
+ * and shifts are unsigned long, column and row-pointer entries int. */
+typedef double vtype;
+typedef int itype;
+typedef unsigned long stype;
+typedef unsigned long gstype;
+typedef long int gsstype;
+
+/* CSRtype<> of src/CSR.h:67-104, the fields with the reference's names,
+ * types and order (the cuSPARSE descriptor, halo_info and rows_to_get of the
+ * MPI exchange are not part of a one-device product and are left out).
+ * Arrays are device pointers when on_the_device, host pointers otherwise.
+ * Columns of a rank's block are in the rank frame (global column -
+ * row_shift). */
 struct CSR {
-  int nnz = 0;                 // entries
-  int n = 0;                   // rows
-  int64_t m = 0;               // columns
-  int shrinked_m = 0;          // columns after compaction
-  int64_t full_n = 0;          // rows of the whole matrix
-  int64_t full_m = 0;          // columns of the whole matrix
+  stype nnz = 0;               // entries
+  stype n = 0;                 // rows
+  gstype m = 0;                // columns
+  stype shrinked_m = 0;        // columns after compaction
+  gstype full_n = 0;           // rows of the whole matrix
+  gstype full_m = 0;           // columns of the whole matrix
   bool on_the_device = false;
+  bool is_symmetric = false;
   bool shrinked_flag = false;
-  int64_t row_shift = 0;       // first global row of the block
-  double *val = nullptr;
-  int *col = nullptr;
-  int *row = nullptr;          // n+1 row pointers
-  int *shrinked_col = nullptr; // compacted columns (device)
-  /* the compaction map of shrink_col: shrinking_permut[c'] = rank-frame
-   * column of compacted column c' (the reference keeps it in a file-static
-   * buffer, src/spspmpi.cu:136-139) */
-  int *shrinking_permut = nullptr;
-  int shrinking_permut_len = 0;
-  /* bounds of the rank-frame columns [col_lo, col_hi) used by shrink_col;
-   * default [-row_shift, m - row_shift) */
+  bool custom_alloced = false;
+  gsstype col_shifted = 0;     // set by the caller (main.cu:208-209), not read here
+  gstype shrinked_firstrow = 0;
+  gstype shrinked_lastrow = 0;
+  vtype *val = nullptr;
+  itype *col = nullptr;
+  itype *row = nullptr;        // n+1 row pointers
+  itype *shrinked_col = nullptr;  // compacted columns (device)
+  gstype row_shift = 0;        // first global row of the block
+  int *bitcol = nullptr;
+  int bitcolsize = 0;
+  int post_local = 0;
+  /* Not in the reference's struct: the compaction map of shrink_col,
+   * shrinking_permut[c'] = rank-frame column of compacted column c' (the
+   * reference keeps it in a file-static buffer, src/spspmpi.cu:136-139),
+   * and the bounds [col_lo, col_hi) of the rank-frame columns shrink_col
+   * flags (default [-row_shift, m - row_shift)). */
+  itype *shrinking_permut = nullptr;
+  stype shrinking_permut_len = 0;
   int64_t col_lo = 0, col_hi = -1;
 };
 
 namespace CSRm {
-CSR *init(int n, int64_t m, int nnz, bool allocate, bool on_the_device, int64_t full_n = -1,
-          int64_t row_shift = 0);
+/* src/CSR.cu:28 (declared at src/CSR.h:114), same parameters: allocate_mem
+ * allocates n+1 row pointers and nnz columns and values on the device or the
+ * host; is_symmetric is recorded (printInfo prints it). */
+CSR *init(stype n, gstype m, stype nnz, bool allocate_mem, bool on_the_device, bool is_symmetric, gstype full_n,
+          gstype row_shift = 0);
 void free(CSR *A);
 /* frees only the struct, not its arrays (src/CSR.cu:624): for views such as
  * get_shrinked_matrix's result */
@@ -114,11 +142,25 @@ CSR *copyToDevice(const CSR *A);
 CSR *copyToHost(const CSR *A);
 /* ||A||_F of a host or device matrix (main.cu:236-241 prints it per rank) */
 double norm_fro(const CSR *A);
-/* The reference's summary of a matrix (src/CSR.cu:631-643, printed by
- * main.cu's printinfo option), same lines and formats; this CSR carries no
- * symmetry flag, so the last line is GENERAL as for every loaded matrix. */
+/* The reference's summary of a matrix (src/CSR.cu:631-643), same lines and
+ * formats, SYMMETRIC or GENERAL from is_symmetric. */
 void printInfo(const CSR *A, FILE *fp = stdout);
 }  // namespace CSRm
+
+/* src/handles.h:5-16: the per-process library state the reference passes to
+ * nsparseMGPU_version1 (its streams and cuSPARSE/cuBLAS handles).  Here it
+ * holds two streams at the default priority, and optionally the full P on
+ * the device: with one device standing in for every rank, the halo rows a
+ * rank would receive over MPI are gathered from it.  Pfull is not owned. */
+struct handles {
+  hipStream_t stream1 = nullptr;
+  hipStream_t stream2 = nullptr;
+  const CSR *Pfull = nullptr;
+};
+namespace Handles {
+handles *init();
+void free(handles *h);
+}  // namespace Handles
 
 /* Compacts A's columns once (src/spspmpi.cu:141-162): perm = sorted
  * unique(cols U [0..l]) with l = P->n - 1 when P is given, else A->n - 1;
@@ -147,6 +189,14 @@ CSR *nsparseMGPU_version0(CSR *A, CSR *P);
  * (CSRm::free). */
 CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local_n = -1, CSR **A_out = nullptr,
                           CSR **P_out = nullptr);
+/* src/spspmpi.cu:371 (called at main.cu:223), same parameters.  Without
+ * h->Pfull, Plocal holds every row of P (one rank) and the product is
+ * nsparseMGPU_version0(Alocal, Plocal), as the reference does when nprocs
+ * is 1 (spspmpi.cu:385).  With h->Pfull, Plocal is the rank's own block of
+ * P (its rows give the local row count, spspmpi.cu:400-404) and the rank's
+ * share is the overload above with Pfull = h->Pfull. */
+CSR *nsparseMGPU_version1(handles *h, CSR *Alocal, CSR *Plocal, double &loc_time);
+CSR *nsparseMGPU_version1(handles *h, CSR *Alocal, CSR *Plocal);
 
 /* Galerkin product C = R * (A * P) on the device, R = P^T when R is NULL
  * (nsp_galerkin): the coarse operator the reference's driver names as its

@@ -62,6 +62,7 @@ def build(force: bool = False, jobs: int = 4) -> str:
         raise RuntimeError("link of libnsp_amd.so failed")
     os.replace(tmp, LIB)
     _build_cpp()
+    build_cpp_tests()
     return LIB
 
 
@@ -87,6 +88,26 @@ def _build_cpp() -> None:
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError(f"build of {os.path.basename(out)} failed")
+        os.replace(out + ".tmp", out)
+
+
+CPP_TESTS = os.path.join(os.path.dirname(HERE), "tests", "cpp")
+
+
+def build_cpp_tests() -> None:
+    """The C++ test programs under tests/cpp (host code written against
+    include/nsp_amd.hpp in the reference's spelling), built with -Wall
+    -Werror next to their sources and linked to the libraries above."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    for src in sorted(glob.glob(os.path.join(CPP_TESTS, "*.cpp"))):
+        out = src[:-4]
+        cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", inc, "-o", out + ".tmp", src,
+               "-L", HERE, "-Wl,-rpath,$ORIGIN/../../nsp_amd",
+               "-lnsp_amd_cpp", "-lnsp_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"build of {os.path.basename(src)} failed")
         os.replace(out + ".tmp", out)
 
 

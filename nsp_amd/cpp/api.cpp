@@ -21,8 +21,9 @@ void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw Error(NSP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// One library handle per host thread and device, on the null stream (the
-// reference's calls are host-synchronous on the legacy stream).
+// One library handle per host thread and device, bound to the legacy (null)
+// stream: the reference's calls are host-synchronous on the legacy stream,
+// so work the caller queued there before a call is ordered before it.
 struct ThreadHandles {
   std::map<int, nsp_handle_t> by_device;
   ~ThreadHandles() {
@@ -39,6 +40,7 @@ nsp_handle_t handle() {
   nsp_handle_t h = nullptr;
   const int s = nsp_create(&h);
   if (s != NSP_OK) throw Error(s, "nsp_create failed");
+  (void)nsp_set_stream(h, nullptr);
   th.by_device[dev] = h;
   return h;
 }
@@ -58,9 +60,9 @@ T *dalloc(size_t n) {
 
 nsp_csr_t view(const CSR *A, bool compacted) {
   nsp_csr_t v;
-  v.rows = A->n;
-  v.cols = compacted ? A->shrinked_m : A->m;
-  v.nnz = A->nnz;
+  v.rows = (int64_t)A->n;
+  v.cols = compacted ? (int64_t)A->shrinked_m : (int64_t)A->m;
+  v.nnz = (int64_t)A->nnz;
   v.row_ptr = A->row;
   v.col = compacted ? A->shrinked_col : A->col;
   v.val = A->val;
@@ -102,24 +104,27 @@ void sfCSR_free_device(sfCSR *c) {
 // ---------------------------------------------------------------------------
 namespace CSRm {
 
-CSR *init(int n, int64_t m, int nnz, bool allocate, bool on_the_device, int64_t full_n, int64_t row_shift) {
+CSR *init(stype n, gstype m, stype nnz, bool allocate_mem, bool on_the_device, bool is_symmetric, gstype full_n,
+          gstype row_shift) {
   CSR *A = new CSR();
   A->n = n;
   A->m = m;
   A->nnz = nnz;
-  A->full_n = full_n < 0 ? n : full_n;
+  A->full_n = full_n;
   A->full_m = m;
   A->row_shift = row_shift;
   A->on_the_device = on_the_device;
-  if (allocate) {
+  A->is_symmetric = is_symmetric;
+  A->shrinked_m = m;
+  if (allocate_mem) {
     if (on_the_device) {
       A->row = dalloc<int>((size_t)n + 1);
-      A->col = dalloc<int>(nnz);
-      A->val = dalloc<double>(nnz);
+      A->col = dalloc<int>((size_t)nnz);
+      A->val = dalloc<double>((size_t)nnz);
     } else {
       A->row = new int[(size_t)n + 1];
-      A->col = new int[std::max(nnz, 1)];
-      A->val = new double[std::max(nnz, 1)];
+      A->col = new int[std::max<size_t>((size_t)nnz, 1)];
+      A->val = new double[std::max<size_t>((size_t)nnz, 1)];
     }
   }
   return A;
@@ -144,7 +149,7 @@ void free(CSR *A) {
 void freeStruct(CSR *A) { delete A; }
 
 CSR *copyToDevice(const CSR *A) {
-  CSR *D = init(A->n, A->m, A->nnz, true, true, A->full_n, A->row_shift);
+  CSR *D = init(A->n, A->m, A->nnz, true, true, A->is_symmetric, A->full_n, A->row_shift);
   D->full_m = A->full_m;
   hip_check(hipMemcpy(D->row, A->row, ((size_t)A->n + 1) * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
   hip_check(hipMemcpy(D->col, A->col, (size_t)A->nnz * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
@@ -153,7 +158,7 @@ CSR *copyToDevice(const CSR *A) {
 }
 
 CSR *copyToHost(const CSR *A) {
-  CSR *H = init(A->n, A->m, A->nnz, true, false, A->full_n, A->row_shift);
+  CSR *H = init(A->n, A->m, A->nnz, true, false, A->is_symmetric, A->full_n, A->row_shift);
   H->full_m = A->full_m;
   hip_check(hipMemcpy(H->row, A->row, ((size_t)A->n + 1) * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
   hip_check(hipMemcpy(H->col, A->col, (size_t)A->nnz * sizeof(int), hipMemcpyDeviceToHost), "hipMemcpy");
@@ -173,14 +178,15 @@ double norm_fro(const CSR *A) {
 }
 
 void printInfo(const CSR *A, FILE *fp) {
-  // src/CSR.cu:631-643 (integers as the reference prints them)
+  // src/CSR.cu:631-643: every size with %d, i.e. the low 32 bits of the
+  // reference's unsigned long fields
   fprintf(fp, "Device?: %d\n", A->on_the_device ? 1 : 0);
-  fprintf(fp, "nnz: %d\n", A->nnz);
-  fprintf(fp, "n: %d\n", A->n);
-  fprintf(fp, "m: %lld\n", (long long)A->m);
-  fprintf(fp, "full_n: %lld\n", (long long)A->full_n);
-  fprintf(fp, "row_shift: %lld\n", (long long)A->row_shift);
-  fprintf(fp, "GENERAL\n");
+  fprintf(fp, "nnz: %d\n", (int)A->nnz);
+  fprintf(fp, "n: %d\n", (int)A->n);
+  fprintf(fp, "m: %d\n", (int)A->m);
+  fprintf(fp, "full_n: %d\n", (int)A->full_n);
+  fprintf(fp, "row_shift: %d\n", (int)A->row_shift);
+  fprintf(fp, A->is_symmetric ? "SYMMETRIC\n" : "GENERAL\n");
   fprintf(fp, "\n");
 }
 
@@ -192,13 +198,14 @@ bool shrink_col(CSR *A, CSR *P) {
   if (!A->on_the_device) throw Error(NSP_ERR_INVALID, "shrink_col: A must be on the device");
   nsp_handle_t h = handle();
   const int64_t l = (int64_t)(P ? P->n : A->n) - 1;
-  const int64_t lo = A->col_hi > A->col_lo ? A->col_lo : -A->row_shift;
-  const int64_t hi = A->col_hi > A->col_lo ? A->col_hi : A->m - A->row_shift;
+  const int64_t shift = (int64_t)A->row_shift;
+  const int64_t lo = A->col_hi > A->col_lo ? A->col_lo : -shift;
+  const int64_t hi = A->col_hi > A->col_lo ? A->col_hi : (int64_t)A->m - shift;
   const int64_t cap = std::max(hi, l + 1) - std::min<int64_t>(lo, 0);
   int *perm = dalloc<int>((size_t)cap);
   int *new_col = dalloc<int>((size_t)A->nnz);
   int64_t m = 0;
-  const int s = nsp_shrink_cols(h, A->nnz, A->col, l, lo, hi, new_col, perm, cap, &m);
+  const int s = nsp_shrink_cols(h, (int64_t)A->nnz, A->col, l, lo, hi, new_col, perm, cap, &m);
   if (s != NSP_OK) {
     (void)hipFree(perm);
     (void)hipFree(new_col);
@@ -206,16 +213,16 @@ bool shrink_col(CSR *A, CSR *P) {
   }
   hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   A->shrinked_col = new_col;
-  A->shrinked_m = (int)m;
+  A->shrinked_m = (stype)m;
   A->shrinking_permut = perm;
-  A->shrinking_permut_len = (int)m;
+  A->shrinking_permut_len = (stype)m;
   A->shrinked_flag = true;
   return true;
 }
 
 CSR *get_shrinked_matrix(CSR *A, CSR *P) {
   shrink_col(A, P);
-  CSR *S = CSRm::init(A->n, A->shrinked_m, A->nnz, false, true, A->full_n, A->row_shift);
+  CSR *S = CSRm::init(A->n, A->shrinked_m, A->nnz, false, A->on_the_device, A->is_symmetric, A->full_n, A->row_shift);
   S->row = A->row;
   S->col = A->shrinked_col;
   S->val = A->val;
@@ -234,7 +241,7 @@ CSR *nsparseMGPU_version0(CSR *A, CSR *P, csrlocinfo * /*Plocal*/, double &loc_t
   check(nsp_spgemm(h, NSP_IDX32, NSP_F64, &a, &b, &c, nullptr, nullptr, nullptr), h, "nsparseMGPU_version0");
   hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   loc_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  CSR *C = CSRm::init(A->n, P->m, (int)c.nnz, false, true, A->full_n, A->row_shift);
+  CSR *C = CSRm::init(A->n, P->m, (stype)c.nnz, false, true, false, A->full_n, A->row_shift);
   C->row = (int *)c.row_ptr;
   C->col = (int *)c.col;
   C->val = (double *)c.val;
@@ -250,31 +257,31 @@ CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local
   if (!Alocal || !Pfull || !Alocal->on_the_device || !Pfull->on_the_device)
     throw Error(NSP_ERR_INVALID, "nsparseMGPU_version1: matrices must be on the device");
   nsp_handle_t h = handle();
-  const int64_t r0 = Alocal->row_shift;
+  const int64_t r0 = (int64_t)Alocal->row_shift;
   CSR Ploc;
-  Ploc.n = p_local_n < 0 ? Alocal->n : p_local_n;
+  Ploc.n = p_local_n < 0 ? Alocal->n : (stype)p_local_n;
   // A's compaction: completedP rows = shrinking_permut(A) + r0
   if (Alocal->col_hi <= Alocal->col_lo) {
     Alocal->col_lo = -r0;
-    Alocal->col_hi = Pfull->n - r0;
+    Alocal->col_hi = (int64_t)Pfull->n - r0;
   }
   CSR *A_ = get_shrinked_matrix(Alocal, &Ploc);
-  const int nrows = Alocal->shrinking_permut_len;
+  const int64_t nrows = (int64_t)Alocal->shrinking_permut_len;
   // completedP: gather of P rows perm + r0, columns re-shifted to the rank frame
   nsp_csr_t pf = view(Pfull, false);
-  CSR *cP = CSRm::init(nrows, Pfull->m, 0, false, true, Pfull->n, r0);
+  CSR *cP = CSRm::init((stype)nrows, Pfull->m, 0, false, true, false, Pfull->n, (gstype)r0);
   cP->row = dalloc<int>((size_t)nrows + 1);
   int64_t nnz = 0;
   check(nsp_gather_rows_count(h, NSP_IDX32, &pf, nrows, Alocal->shrinking_permut, r0, cP->row, &nnz), h,
         "completedP rows");
-  cP->nnz = (int)nnz;
+  cP->nnz = (stype)nnz;
   cP->col = dalloc<int>((size_t)nnz);
   cP->val = dalloc<double>((size_t)nnz);
   nsp_csr_t out = view(cP, false);
   check(nsp_gather_rows_fill(h, NSP_IDX32, NSP_F64, &pf, nrows, Alocal->shrinking_permut, r0, r0, &out), h,
         "completedP fill");
   cP->col_lo = -r0;
-  cP->col_hi = Pfull->m - r0;
+  cP->col_hi = (int64_t)Pfull->m - r0;
   CSR *P_ = get_shrinked_matrix(cP, nullptr);
   CSR *C = nsparseMGPU_version0(A_, P_, nullptr, loc_time);
   if (A_out) {
@@ -296,6 +303,40 @@ CSR *nsparseMGPU_version1(CSR *Alocal, CSR *Pfull, double &loc_time, int p_local
   return C;
 }
 
+CSR *nsparseMGPU_version1(handles *hs, CSR *Alocal, CSR *Plocal, double &loc_time) {
+  if (!hs || !Alocal || !Plocal) throw Error(NSP_ERR_INVALID, "nsparseMGPU_version1: NULL argument");
+  if (!hs->Pfull) {
+    // one rank: Plocal holds every row of P (spspmpi.cu:385)
+    csrlocinfo info{0, (int)Plocal->n, Plocal->row, Plocal->col, Plocal->val};
+    return nsparseMGPU_version0(Alocal, Plocal, &info, loc_time);
+  }
+  if (Plocal->row_shift != Alocal->row_shift)
+    throw Error(NSP_ERR_INVALID, "nsparseMGPU_version1: Alocal and Plocal start at different rows");
+  if (Plocal->n > (stype)INT32_MAX) throw Error(NSP_ERR_OVERFLOW, "nsparseMGPU_version1: Plocal->n >= 2^31");
+  return nsparseMGPU_version1(Alocal, const_cast<CSR *>(hs->Pfull), loc_time, (int)Plocal->n);
+}
+
+CSR *nsparseMGPU_version1(handles *hs, CSR *Alocal, CSR *Plocal) {
+  double t = 0.0;
+  return nsparseMGPU_version1(hs, Alocal, Plocal, t);
+}
+
+namespace Handles {
+handles *init() {
+  handles *h = new handles();
+  hip_check(hipStreamCreate(&h->stream1), "hipStreamCreate");
+  hip_check(hipStreamCreate(&h->stream2), "hipStreamCreate");
+  return h;
+}
+
+void free(handles *h) {
+  if (!h) return;
+  if (h->stream1) (void)hipStreamDestroy(h->stream1);
+  if (h->stream2) (void)hipStreamDestroy(h->stream2);
+  delete h;
+}
+}  // namespace Handles
+
 // ---------------------------------------------------------------------------
 CSR *galerkin_product(CSR *R, CSR *A, CSR *P) {
   if (!A || !P || !A->on_the_device || !P->on_the_device || (R && !R->on_the_device))
@@ -308,9 +349,9 @@ CSR *galerkin_product(CSR *R, CSR *A, CSR *P) {
         "galerkin_product");
   if (c.nnz > INT32_MAX) {
     nsp_csr_free(&c);
-    throw Error(NSP_ERR_OVERFLOW, "galerkin_product: nnz does not fit CSR's int");
+    throw Error(NSP_ERR_OVERFLOW, "galerkin_product: nnz does not fit CSR's int row pointers");
   }
-  CSR *C = CSRm::init((int)c.rows, c.cols, (int)c.nnz, false, true, c.rows, 0);
+  CSR *C = CSRm::init((stype)c.rows, (gstype)c.cols, (stype)c.nnz, false, true, false, (gstype)c.rows, 0);
   C->row = (int *)c.row_ptr;
   C->col = c.col;
   C->val = (double *)c.val;
@@ -352,7 +393,7 @@ CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type,
   // rows first to last present row (distribute.cu:1094), columns shifted by
   // the rows of the ranks before (distribute.cu:1095, 1103)
   const int n = (int)s.nrows, nnz = (int)nrec;
-  CSR *A = CSRm::init(n, s.nc, nnz, true, false, s.nr, s.col_shift);
+  CSR *A = CSRm::init((stype)n, (gstype)s.nc, (stype)nnz, true, false, false, (gstype)s.nr, (gstype)s.col_shift);
   std::fill(A->row, A->row + n + 1, 0);
   for (const CooRec &r : rec) {
     const int64_t x = r.i - 1 - s.first_row;
@@ -391,7 +432,7 @@ CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type,
 CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type) {
   const nsp_coo_share_t s = coo_share(path, nprocs, rank, header_type, "load_MatrixMPI");
   const int n = (int)s.nrows, nnz = (int)(s.rec_end - s.rec_begin);
-  CSR *A = CSRm::init(n, s.nc, nnz, true, true, s.nr, s.col_shift);
+  CSR *A = CSRm::init((stype)n, (gstype)s.nc, (stype)nnz, true, true, false, (gstype)s.nr, (gstype)s.col_shift);
   nsp_handle_t h = handle();
   nsp_csr_t out{n, s.nc, nnz, A->row, A->col, A->val};
   const int st = nsp_coo_read(h, NSP_IDX32, NSP_F64, path, header_type, &s, &out);
@@ -411,8 +452,8 @@ void CSRMatrixPrintMM(CSR *A_, const char *file_name) {
   }
   // the reference's formats (matrixIO.cu:328-336), 1-based indices
   fprintf(fp, "%s\n", "%%MatrixMarket matrix coordinate real general");
-  fprintf(fp, "%d  %lld %d \n", A->n, (long long)A->m, A->nnz);
-  for (int i = 0; i < A->n; i++)
+  fprintf(fp, "%d  %d %d \n", (int)A->n, (int)A->m, (int)A->nnz);
+  for (int i = 0; i < (int)A->n; i++)
     for (int j = A->row[i]; j < A->row[i + 1]; j++) fprintf(fp, "%d   %d  %lg\n", i + 1, A->col[j] + 1, A->val[j]);
   const bool ok = !ferror(fp);
   fclose(fp);
@@ -434,11 +475,11 @@ void write_binary_matrix(const char *path, const CSR *A, int64_t col_shift) {
   if (A->on_the_device) throw Error(NSP_ERR_INVALID, "write_binary_matrix: host matrix expected");
   FILE *f = fopen(path, "wb");
   if (!f) throw Error(NSP_ERR_INVALID, std::string("write_binary_matrix: cannot open ") + path);
-  const int64_t h[3] = {A->n, A->m, A->nnz};
+  const int64_t h[3] = {(int64_t)A->n, (int64_t)A->m, (int64_t)A->nnz};
   bool ok = fwrite(h, 8, 3, f) == 3;
   std::vector<CooRec> buf;
   buf.reserve(1 << 16);
-  for (int i = 0; i < A->n && ok; i++) {
+  for (int i = 0; i < (int)A->n && ok; i++) {
     for (int p = A->row[i]; p < A->row[i + 1]; p++) {
       buf.push_back(CooRec{(int64_t)i + 1, (int64_t)A->col[p] + col_shift + 1, A->val[p]});
       if (buf.size() == buf.capacity()) {

@@ -51,13 +51,13 @@ static int run_product(int argc, char **argv) {
   sfCSR a{}, b{}, c{};
   a.M = (unsigned)Ad->n;
   a.N = (unsigned)Ad->m;
-  a.nnz = Ad->nnz;
+  a.nnz = (int)Ad->nnz;
   a.d_rpt = Ad->row;
   a.d_col = Ad->col;
   a.d_val = Ad->val;
   b.M = (unsigned)Bd->n;
   b.N = (unsigned)Bd->m;
-  b.nnz = Bd->nnz;
+  b.nnz = (int)Bd->nnz;
   b.d_rpt = Bd->row;
   b.d_col = Bd->col;
   b.d_val = Bd->val;
@@ -65,9 +65,9 @@ static int run_product(int argc, char **argv) {
   nsp_spgemm_kernel_hash(&a, &b, &c);
   const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   CSR Cv;
-  Cv.n = (int)c.M;
+  Cv.n = c.M;
   Cv.m = c.N;
-  Cv.nnz = c.nnz;
+  Cv.nnz = (stype)c.nnz;
   Cv.row = c.d_rpt;
   Cv.col = c.d_col;
   Cv.val = c.d_val;
@@ -107,12 +107,12 @@ static int run_local(int argc, char **argv) {
   // main.cu:241-247: the summary line and, when asked, the outtime.txt line
   // (one rank: loc_time is also the total)
   printf("TOTAL_TIME: %f\t\tlocal time: %f\t\tnnz = %lu\t\tl2norm: %f\n", loc_time, loc_time, (unsigned long)C->nnz, l2);
-  if (argc > 7) append_outtime(argv[7], (uint64_t)C->n, C->m, (uint64_t)C->nnz, loc_time, loc_time, l2);
-  printf("{\"mode\": \"local\", \"nprocs\": %d, \"rank\": %d, \"row_shift\": %lld, \"rows\": %d, "
-         "\"shrinked_m_A\": %d, \"completedP_rows\": %d, \"shrinked_m_P\": %d, \"nnz\": %d, "
+  if (argc > 7) append_outtime(argv[7], (uint64_t)C->n, (int64_t)C->m, (uint64_t)C->nnz, loc_time, loc_time, l2);
+  printf("{\"mode\": \"local\", \"nprocs\": %d, \"rank\": %d, \"row_shift\": %lu, \"rows\": %lu, "
+         "\"shrinked_m_A\": %lu, \"completedP_rows\": %lu, \"shrinked_m_P\": %lu, \"nnz\": %lu, "
          "\"l2norm\": %.17g, \"loc_time\": %.6f, \"read_s\": %.6f}\n",
-         nprocs, rank, (long long)Ad->row_shift, C->n, A_->shrinked_m ? A_->shrinked_m : (int)A_->m, P_->n,
-         (int)P_->m, C->nnz, l2, loc_time, read_s);
+         nprocs, rank, Ad->row_shift, C->n, (unsigned long)A_->m, P_->n, (unsigned long)P_->m, C->nnz, l2, loc_time,
+         read_s);
   CSRm::free(C);
   CSRm::free(P_);
   CSRm::freeStruct(A_);
@@ -125,7 +125,7 @@ static int run_mm(int argc, char **argv) {
   if (argc < 4) return 2;
   CSR *A = read_binary_matrix(argv[2], 1, 0);
   CSRMatrixPrintMM(A, argv[3]);
-  printf("{\"mode\": \"mm\", \"rows\": %d, \"nnz\": %d}\n", A->n, A->nnz);
+  printf("{\"mode\": \"mm\", \"rows\": %lu, \"nnz\": %lu}\n", A->n, A->nnz);
   CSRm::free(A);
   return 0;
 }
@@ -139,9 +139,9 @@ static int run_galerkin(int argc, char **argv) {
   const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   const double l2 = CSRm::norm_fro(C);
   if (argc > 4) write_device_csr(argv[4], C);
-  printf("{\"mode\": \"galerkin\", \"rows\": %d, \"cols\": %lld, \"nnz\": %d, \"l2norm\": %.17g, "
+  printf("{\"mode\": \"galerkin\", \"rows\": %lu, \"cols\": %lu, \"nnz\": %lu, \"l2norm\": %.17g, "
          "\"time_s\": %.6f}\n",
-         C->n, (long long)C->m, C->nnz, l2, t);
+         C->n, C->m, C->nnz, l2, t);
   CSRm::free(C);
   CSRm::free(A);
   CSRm::free(P);
@@ -165,18 +165,18 @@ static int run_split(int argc, char **argv, bool device) {
     nc = s.nc;
     nt = s.nt;
     write_binary_matrix(argv[5], A, 0);
-    printf("{\"mode\": \"load\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %d, \"nnz\": %d, "
-           "\"row_shift\": %lld, \"read_s\": %.6f}\n",
-           (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, (long long)A->row_shift, td);
+    printf("{\"mode\": \"load\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %lu, \"nnz\": %lu, "
+           "\"row_shift\": %lu, \"read_s\": %.6f}\n",
+           (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, A->row_shift, td);
     CSRm::free(A);
     return 0;
   }
   A = read_binary_matrix(argv[2], nprocs, rank, 1, &nr, &nc, &nt);
   const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   write_binary_matrix(argv[5], A, 0);
-  printf("{\"mode\": \"split\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %d, \"nnz\": %d, "
-         "\"row_shift\": %lld, \"read_s\": %.6f}\n",
-         (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, (long long)A->row_shift, t);
+  printf("{\"mode\": \"split\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %lu, \"nnz\": %lu, "
+         "\"row_shift\": %lu, \"read_s\": %.6f}\n",
+         (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, A->row_shift, t);
   CSRm::free(A);
   return 0;
 }

@@ -246,8 +246,7 @@ __device__ __forceinline__ int table_log(int64_t nz) {
 // table column -> dC sized for the row; lane (i,k) walks B row k in order and
 // sums B_kj * dC_ij (deterministic).  A launch takes rows [0, MA) (row list
 // NULL) or the rows of a device list; rows whose C row passes NS entries are
-// appended to long_rows for the next tier, or set *too_long when there is no
-// next tier.
+// appended to long_rows for the next tier.
 template <typename V, typename RowT, int GS, int NS>
 __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const int32_t *__restrict__ rows,
                                                 const unsigned int *__restrict__ nrows,
@@ -259,8 +258,7 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const int32_t *__res
                                                 const V *__restrict__ dc, V *__restrict__ da,
                                                 const V *__restrict__ aval, const int64_t *__restrict__ poff,
                                                 V *__restrict__ w,
-                                                int32_t *__restrict__ long_rows, unsigned int *__restrict__ nlong,
-                                                uint32_t *__restrict__ too_long) {
+                                                int32_t *__restrict__ long_rows, unsigned int *__restrict__ nlong) {
   constexpr int RPB = 256 / GS;
   constexpr int SH = 2 * NS;
   __shared__ int32_t keys[RPB][SH];
@@ -273,10 +271,7 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const int32_t *__res
     const int64_t i = rows ? rows[t] : t;
     const int64_t c0 = (int64_t)crpt[i], nz = (int64_t)crpt[i + 1] - c0;
     if (nz > NS) {
-      if (glane == 0) {
-        if (long_rows) long_rows[atomicAdd(nlong, 1u)] = (int32_t)i;
-        else *too_long = 1;
-      }
+      if (glane == 0) long_rows[atomicAdd(nlong, 1u)] = (int32_t)i;
       continue;
     }
     const int log = table_log(nz);
@@ -313,6 +308,51 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const int32_t *__res
       da[p] = s;
     }
     wave_lds_sync();
+  }
+}
+
+// dA for rows whose C row passes the last table tier (more than 1024
+// entries): one wavefront per listed row, one lane per A entry (i,k), and
+// dC_ij found by binary search in the sorted C row i (no table, so no bound
+// on the row length).  Same sums in the same order as k_grad_a.
+template <typename V, typename RowT>
+__global__ __launch_bounds__(256) void k_grad_a_search(const int32_t *__restrict__ rows,
+                                                       const unsigned int *__restrict__ nrows,
+                                                       const RowT *__restrict__ arpt,
+                                                       const int64_t *__restrict__ ab_b0,
+                                                       const int32_t *__restrict__ ab_len,
+                                                       const int32_t *__restrict__ bcol, const V *__restrict__ bval,
+                                                       const RowT *__restrict__ crpt, const int32_t *__restrict__ ccol,
+                                                       const V *__restrict__ dc, V *__restrict__ da,
+                                                       const V *__restrict__ aval, const int64_t *__restrict__ poff,
+                                                       V *__restrict__ w) {
+  const int lane = threadIdx.x & 63;
+  const int64_t count = (int64_t)*nrows;
+  for (int64_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < count; t += (int64_t)gridDim.x * 4) {
+    const int64_t i = rows[t];
+    const int64_t c0 = (int64_t)crpt[i], c1 = (int64_t)crpt[i + 1];
+    const int64_t a0 = (int64_t)arpt[i], a1 = (int64_t)arpt[i + 1];
+    for (int64_t p = a0 + lane; p < a1; p += 64) {
+      const int64_t b0 = ab_b0[p];
+      const int len = ab_len[p];
+      const V a = w ? aval[p] : (V)0;
+      V s = (V)0;
+      int64_t lo = c0;  // B row k is sorted: each search starts at the previous hit
+      for (int j = 0; j < len; j++) {
+        const int32_t key = bcol[b0 + j];
+        int64_t l = lo, r = c1;
+        while (l < r) {
+          const int64_t mid = (l + r) >> 1;
+          if (ccol[mid] < key) l = mid + 1; else r = mid;
+        }
+        // key is in C row i whenever A_ik and B_kj are stored
+        const V d = (l < c1 && ccol[l] == key) ? dc[l] : (V)0;
+        lo = l;
+        s += bval[b0 + j] * d;
+        if (w) w[poff[p] + j] = a * d;
+      }
+      da[p] = s;
+    }
   }
 }
 
@@ -434,7 +474,7 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
   const bool contrib = at_perm && nprod > 0 && (size_t)nprod * sizeof(V) <= kBackwardContribMax;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = nnzA / kScanTile + 2;
-  const size_t o_rows = 256, o_status = o_rows + 2 * al((size_t)std::max<int64_t>(MA, 1) * 4),
+  const size_t o_rows = 256, o_status = o_rows + 3 * al((size_t)std::max<int64_t>(MA, 1) * 4),
                o_poff = o_status + al((size_t)ntiles * 8), o_w = o_poff + al((size_t)(nnzA + 1) * 8),
                need = contrib ? o_w + al((size_t)nprod * sizeof(V)) : o_poff;
   if (need > h->bwd.bytes) {
@@ -449,13 +489,16 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
     h->bwd.bytes = need + need / 8;
   }
   char *base = (char *)h->bwd.base;
-  // [0] too long, [1] rows for the second tier, [2] scan ticket, [3] rows for the third tier
+  // [1] rows for the second tier, [2] scan ticket, [3] rows for the third
+  // tier, [4] rows for the search tier
   uint32_t *flags = (uint32_t *)base;
+  const size_t list_bytes = al((size_t)std::max<int64_t>(MA, 1) * 4);
   int32_t *mid_rows = (int32_t *)(base + o_rows);
-  int32_t *long_rows = (int32_t *)(base + o_rows + al((size_t)std::max<int64_t>(MA, 1) * 4));
+  int32_t *long_rows = (int32_t *)(base + o_rows + list_bytes);
+  int32_t *search_rows = (int32_t *)(base + o_rows + 2 * list_bytes);
   int64_t *poff = contrib ? (int64_t *)(base + o_poff) : nullptr;
   V *w = contrib ? (V *)(base + o_w) : nullptr;
-  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 4);
+  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 5);
   if (contrib && !h->poff_valid) {
     // product offsets of the A entries: exclusive scan of the B row lengths
     uint64_t *status = (uint64_t *)(base + o_status);
@@ -468,23 +511,26 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
   if (MA > 0) {
     // tier 1: C rows up to 32 entries (32 lanes per row, 64-slot tables);
     // longer rows go to a device list for tier 2 (up to 128 entries), whose
-    // longer rows go to one for tier 3 (up to 1024 entries, 64 lanes).  The
-    // small first tier keeps the LDS per block low (occupancy) for the
-    // common short C rows.
+    // longer rows go to one for tier 3 (up to 1024 entries, 64 lanes), whose
+    // longer rows go to the search tier (any length).  The small first tier
+    // keeps the LDS per block low (occupancy) for the common short C rows.
     const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 8));
     hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 32>), dim3(ga), dim3(256), 0, st, MA, (const int32_t *)nullptr,
                        (const unsigned int *)nullptr, (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
                        (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
-                       (const int64_t *)poff, w, mid_rows, flags + 1, (uint32_t *)nullptr);
+                       (const int64_t *)poff, w, mid_rows, flags + 1);
     hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128>), dim3(h->num_cu * 2), dim3(256), 0, st, MA,
                        (const int32_t *)mid_rows, (const unsigned int *)(flags + 1), (const RowT *)A->row_ptr,
                        h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
-                       (const V *)A->val, (const int64_t *)poff, w, long_rows, flags + 3, (uint32_t *)nullptr);
+                       (const V *)A->val, (const int64_t *)poff, w, long_rows, flags + 3);
     hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024>), dim3(h->num_cu), dim3(256), 0, st, MA,
                        (const int32_t *)long_rows, (const unsigned int *)(flags + 3), (const RowT *)A->row_ptr,
                        h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
-                       (const V *)A->val, (const int64_t *)poff, w, (int32_t *)nullptr, (unsigned int *)nullptr,
-                       flags);
+                       (const V *)A->val, (const int64_t *)poff, w, search_rows, flags + 4);
+    hipLaunchKernelGGL((k_grad_a_search<V, RowT>), dim3(h->num_cu), dim3(256), 0, st, (const int32_t *)search_rows,
+                       (const unsigned int *)(flags + 4), (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
+                       (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
+                       (const int64_t *)poff, w);
   }
   if (B->rows > 0 && B->nnz > 0) {
     // lanes per B entry from the mean A^T row length
@@ -508,15 +554,6 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
 #undef GRAD_B
   }
   NSP_HIP(h, hipGetLastError());
-  // C's row lengths are checked once per C pattern: with fixed patterns
-  // (reuse_extents) later calls on the same C buffers need no host sync
-  if (h->reuse_extents && h->bwd_c_key[0] == C->row_ptr && h->bwd_c_key[1] == C->col) return NSP_OK;
-  uint32_t tl = 0;
-  NSP_HIP(h, hipMemcpyAsync(&tl, flags, 4, hipMemcpyDeviceToHost, st));
-  NSP_HIP(h, hipStreamSynchronize(st));
-  if (tl) return set_error(h, NSP_ERR_UNSUPPORTED, "backward: C rows with more than 1024 entries");
-  h->bwd_c_key[0] = C->row_ptr;
-  h->bwd_c_key[1] = C->col;
   return NSP_OK;
 }
 

@@ -51,11 +51,15 @@ __global__ void k_shrink_mark(int64_t nnz, const int32_t *__restrict__ col, int6
   }
 }
 
-__global__ void k_shrink_remap(int64_t nnz, const int32_t *__restrict__ col, int64_t base,
+// A column outside the flagged range (already reported by k_shrink_mark)
+// maps to -1 instead of indexing past the position array.
+__global__ void k_shrink_remap(int64_t nnz, const int32_t *__restrict__ col, int64_t base, int64_t range,
                                const int32_t *__restrict__ pos, int32_t *__restrict__ new_col) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
-       e += (int64_t)gridDim.x * blockDim.x)
-    new_col[e] = pos[(int64_t)col[e] - base];
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = (int64_t)col[e] - base;
+    new_col[e] = (x >= 0 && x < range) ? pos[x] : -1;
+  }
 }
 
 __global__ void k_shrink_perm(int64_t range, int64_t base, const uint8_t *__restrict__ flags,
@@ -164,7 +168,8 @@ int shrink_cols(Handle *h, int64_t nnz, const int32_t *col, int64_t l, int64_t l
   hipLaunchKernelGGL((k_exscan<uint8_t, int32_t>), dim3((unsigned)(range / kScanTile + 1)), dim3(kScanBS), 0,
                      st, range, flags, pos, sc.status, sc.ticket);
   if (nnz > 0)
-    hipLaunchKernelGGL(k_shrink_remap, dim3(grid1d(h, nnz)), dim3(256), 0, st, nnz, col, base, pos, new_col);
+    hipLaunchKernelGGL(k_shrink_remap, dim3(grid1d(h, nnz)), dim3(256), 0, st, nnz, col, base, range, pos,
+                       new_col);
   hipLaunchKernelGGL(k_shrink_perm, dim3(grid1d(h, range)), dim3(256), 0, st, range, base, flags, pos, perm,
                      perm_cap);
   NSP_HIP(h, hipGetLastError());

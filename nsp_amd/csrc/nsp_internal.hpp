@@ -91,7 +91,6 @@ struct Handle {
   size_t pin_bytes = 0;
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   bool poff_valid = false;  // bwd holds the product offsets of the cached extents
-  const void *bwd_c_key[2] = {nullptr, nullptr};  // C pattern whose rows passed the length check
   Info *d_info = nullptr;
   Info *h_info = nullptr;  // pinned
 

@@ -144,20 +144,24 @@ def _mixed(nrows, ncols, lo, hi, seed):
     return gen._from_rows_cols(nrows, ncols, rows, cols, rng.uniform(-1, 1, rows.size))
 
 
-@pytest.mark.parametrize("case", ["stencil", "mixed"])
+@pytest.mark.parametrize("case", ["stencil", "mixed", "rmat"])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("use_perm", [False, True])
 def test_backward_general_patterns(nsp, orc, case, dtype, use_perm):
-    """A*A on the 27-point stencil (C rows up to 125 entries) and a product of
-    mixed row lengths (C rows past 128, the second table tier), with A's
-    values given as A^T values or through the A^T -> A permutation."""
+    """A*A on the 27-point stencil (C rows up to 125 entries), a product of
+    mixed row lengths (C rows past 128, the second table tier) and R-MAT
+    scale 14 A*A (6244 C rows past 1024 entries, up to 10,358: the search
+    tier), with A's values given as A^T values or through the A^T -> A
+    permutation."""
     print("case: backward general", case, dtype, use_perm, flush=True)
     import torch
     if case == "stencil":
         A = B = to_dtype(gen.stencil_27pt(10, seed=4), dtype)
-    else:
+    elif case == "mixed":
         A = to_dtype(_mixed(600, 500, 1, 40, seed=5), dtype)
         B = to_dtype(_mixed(500, 700, 1, 30, seed=6), dtype)
+    else:
+        A = B = to_dtype(gen.rmat(14, seed=3), dtype)
     h = nsp.Handle()
     dA_ = nsp.DeviceCSR.from_host(A, dtype)
     dB_ = nsp.DeviceCSR.from_host(B, dtype)
@@ -165,6 +169,8 @@ def test_backward_general_patterns(nsp, orc, case, dtype, use_perm):
     crpt = C.row_ptr.cpu().numpy().astype(np.int64)
     if case == "mixed":
         assert np.diff(crpt).max() > 128
+    if case == "rmat":
+        assert (np.diff(crpt) > 1024).sum() > 1000
     if use_perm:
         idx = torch.arange(A.nnz, dtype=torch.float64, device="cuda")
         ATi = h.transpose(nsp.DeviceCSR(dA_.row_ptr, dA_.col, idx, A.nrows, A.ncols))

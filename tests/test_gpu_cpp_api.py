@@ -149,3 +149,44 @@ def test_local_prints_reference_summary_and_outtime(orc, tmp_path):
     assert line.endswith("\n") and len(line) == 3 * 16 + (10 + 1 + 10 + 1 + 50 + 1)
     assert line[:48] == "%15d %15d %15d " % (meta["rows"], A.ncols, meta["nnz"])
     assert abs(float(line[-51:]) - meta["l2norm"]) <= 1e-6 * meta["l2norm"]
+
+
+MAIN_SEQ = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "main_sequence")
+
+
+@pytest.mark.parametrize("nprocs,rank", [(1, 0), (4, 0), (4, 1), (4, 2), (4, 3)])
+def test_reference_main_sequence(orc, tmp_path, nprocs, rank):
+    """tests/cpp/main_sequence.cpp makes main.cu's calls in the reference's
+    own spelling against include/nsp_amd.hpp (compiled with -Wall -Werror):
+    CSRm::init with is_symmetric and the row shift as distribute.cu:1107
+    passes them, col_shifted set as main.cu:208-209, the handles overload
+    nsparseMGPU_version1(h, Alocal, Plocal, loc_time) of main.cu:223, and
+    nnz/n read as unsigned long (main.cu:236-237).  27-point stencil 16^3 as
+    A and P; C against the oracle's local share product."""
+    print("case: reference main sequence", nprocs, rank, flush=True)
+    m = 16
+    A = gen.stencil_27pt(m, seed=9)
+    pa, pc = str(tmp_path / "a.bin"), str(tmp_path / "c.bin")
+    orc.write_binary_coo(pa, A)
+    r = subprocess.run([MAIN_SEQ, pa, pa, str(nprocs), str(rank), pc, "1"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    meta = json.loads(r.stdout.strip().splitlines()[-1])
+    nr, nc, nt, r0, A_rank = orc.read_binary_coo(pa, nprocs, rank)
+    _, _, _, _, P_rank = orc.read_binary_coo(pa, nprocs, rank)
+    # printInfo of Alocal and Plocal (src/CSR.cu:631-643)
+    block = (f"Device?: 1\nnnz: {A_rank.nnz}\nn: {A_rank.nrows}\nm: {nc}\nfull_n: {nr}\nrow_shift: {r0}\n"
+             f"GENERAL\n\n")
+    assert r.stdout.split("TOTAL_TIME")[0] == block + block
+    if nprocs == 1:
+        ref = orc.spgemm(A, A)
+        ncols = A.ncols
+    else:
+        sh = orc.local_share(A_rank, A, r0, P_rank.nrows)
+        ref = orc.spgemm(sh["A_"], sh["P_"])
+        ncols = sh["P_"].ncols
+    assert (meta["tot_n"], meta["tot_nnz"], meta["m"]) == (A_rank.nrows, int(ref[0][-1]), ncols)
+    assert (meta["row_shift"], meta["full_n"]) == (r0, nr)
+    rpt, col, val, _ = read_c(pc)
+    compare(rpt, col, val, ref)
+    assert abs(meta["l2norm"] - np.linalg.norm(ref[2])) <= 1e-10 * np.linalg.norm(ref[2])

@@ -152,9 +152,11 @@ def test_rmat_scale16_sampled(nsp, orc):
 
 
 def test_rmat_scale20_sampled(nsp, orc):
-    """R-MAT scale 20 A*A in FP64 (nnz(C) ~ 8.5e9, int64 row pointers): the
-    heaviest rows and a seeded sample against the oracle, every row strictly
-    increasing, nnz(C) equal to the sum of the row counts."""
+    """R-MAT scale 20 A*A in FP64 (nnz(C) ~ 9.7e9, int64 row pointers): the
+    whole row pointer (1,048,577 entries) and nnz(C) bit-exact against the
+    exclusive scan of the oracle's per-row symbolic counts, the heaviest rows
+    and a seeded sample against the oracle's values, every row strictly
+    increasing."""
     print("case: rmat 20", flush=True)
     import torch
     A = gen.rmat(20, seed=12345)
@@ -166,6 +168,12 @@ def test_rmat_scale20_sampled(nsp, orc):
     print(f"  nprod {st['nprod']} nnz {st['nnz']} heavy rows sym {st['sym_bin'][8]} num {st['num_bin'][8]}",
           flush=True)
     assert st["nprod"] == nprod and st["nnz"] > 2**31
+    ref_rpt = orc.exscan(orc.symbolic(A, A))
+    got_rpt = C.row_ptr.cpu().numpy()
+    assert got_rpt.shape == ref_rpt.shape
+    np.testing.assert_array_equal(got_rpt, ref_rpt, err_msg="row_ptr differs from the oracle's counts")
+    assert C.nnz == int(ref_rpt[-1]) == st["nnz"]
+    print(f"  row_ptr of all {A.nrows} rows bit-exact, nnz(C) {C.nnz}", flush=True)
     rng = np.random.default_rng(0)
     heavy = np.argsort(ip)[-6:]
     sample = np.unique(np.concatenate([heavy, rng.choice(A.nrows, 60, replace=False)]))

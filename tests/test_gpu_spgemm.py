@@ -216,6 +216,30 @@ def test_stencil_rank0_share_identity(nsp, orc):
     np.testing.assert_array_equal(C.col.cpu().numpy(), ccol)
 
 
+def test_shrink_cols_rejects_out_of_range_column(nsp, orc):
+    """A column outside the promised [lo, hi) is reported as NSP_ERR_INVALID;
+    the remap kernel maps it to -1 instead of reading past its position
+    array, and every in-range column is still remapped (checked against the
+    oracle's shrink map on the valid entries)."""
+    print("case: shrink out-of-range column", flush=True)
+    import torch
+    rng = np.random.default_rng(7)
+    col = rng.integers(-50, 400, 5000).astype(np.int32)
+    col[1234] = 2_000_000_000          # far outside [lo, hi)
+    col[4321] = -2_000_000_000
+    dcol = torch.from_numpy(col).cuda()
+    h = nsp.Handle()
+    with pytest.raises(nsp.NspError) as ei:
+        h.shrink_cols(dcol, 99, -50, 400)
+    assert ei.value.status == nsp._lib.NSP_ERR_INVALID
+    # the handle stays usable and the in-range map is right
+    ok = np.delete(col, [1234, 4321])
+    new_col, perm = h.shrink_cols(torch.from_numpy(ok).cuda(), 99, -50, 400)
+    rperm, rnew = orc.shrink(ok, 99)
+    np.testing.assert_array_equal(perm.cpu().numpy(), rperm)
+    np.testing.assert_array_equal(new_col.cpu().numpy(), rnew)
+
+
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_wide_columns_unpacked_keys(nsp, orc, dtype):
     """C rows of 257-512 entries (numeric tables of 1024 slots) over 5 M
