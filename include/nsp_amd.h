@@ -145,6 +145,29 @@ int nsp_spgemm(nsp_handle_t h, nsp_index_t it, nsp_value_t vt,
                const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C,
                nsp_alloc_fn alloc, nsp_free_fn dealloc, void *user);
 
+/* Bytes of the device array that nsp_spgemm_noalloc uses as its scratch
+ * (one copy per device, held until the process ends). */
+#define NSP_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* C = A*B with no allocation: no hipMalloc, hipFree or allocating library
+ * call.  No counterpart in the reference; for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The caller passes C->row_ptr
+ * (A->rows+1 entries) and C->col / C->val with a capacity of C->nnz
+ * entries; the per-call counters and the workspace live in the library's
+ * static device array of NSP_NOALLOC_SCRATCH_BYTES (A needs about
+ * 37 bytes per row + 12 bytes per entry of it; larger A is rejected with
+ * NSP_ERR_UNSUPPORTED before any work).  The kernels are those of
+ * nsp_spgemm, launched for every bin with the row counts read on the device,
+ * so the call issues no host synchronisation between them.  With nnz
+ * non-NULL it waits for the stream and returns nnz(C); rows past the
+ * capacity are skipped and NSP_ERR_OVERFLOW returned with *nnz = the size C
+ * needs.  With nnz NULL it only enqueues the work (nnz(C) is then
+ * C->row_ptr[A->rows]).  Every call on one device shares that device's
+ * array: calls that overlap on one device (other streams or threads) race. */
+int nsp_spgemm_noalloc(nsp_handle_t h, nsp_index_t it, nsp_value_t vt,
+                       const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C, int64_t *nnz);
+
 /* Frees a C returned by nsp_spgemm with the default allocator
  * (CSRm::free, src/CSR.cu:572-621). */
 int nsp_csr_free(nsp_csr_t *C);

@@ -60,6 +60,7 @@ SIGNATURES = [
     ("nsp_spgemm_symbolic", _I, [_H, _I, _I, _CSRP, _CSRP, _VP, _P64]),
     ("nsp_spgemm_numeric", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP]),
     ("nsp_spgemm", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _VP, _VP, _VP]),
+    ("nsp_spgemm_noalloc", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _P64]),
     ("nsp_csr_free", _I, [_CSRP]),
     ("nsp_intprod", _I, [_H, _I, _CSRP, _CSRP, _VP, _P64]),
     ("nsp_check_csr", _I, [_H, _I, _CSRP, _P64]),

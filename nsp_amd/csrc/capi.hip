@@ -96,6 +96,10 @@ int nsp_create(nsp_handle_t *out) {
     return NSP_ERR_OOM;
   }
   std::memset(h->h_info, 0, sizeof(nsp::Info));
+  // device address of the no-allocation product's scratch array on this
+  // device (a failure here is reported again by nsp_spgemm_noalloc)
+  (void)nsp::noalloc_setup(h);
+  h->last_error.clear();
   *out = hs;
   return NSP_OK;
 }
@@ -220,6 +224,12 @@ static void *default_alloc(size_t bytes, void *) {
 }
 static void default_free(void *p, void *) {
   if (p) (void)hipFree(p);
+}
+
+int nsp_spgemm_noalloc(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                       nsp_csr_t *C, int64_t *nnz) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::spgemm_noalloc(&hs->h, it, vt, A, B, C, nnz);
 }
 
 int nsp_spgemm(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "device_utils.hpp"
 #include "nsp_internal.hpp"
@@ -1159,11 +1161,18 @@ int collect_timing(Handle *h) {
 }
 
 // ---------------------------------------------------------------------------
+// Bytes of the product workspace for A with MA rows and nnzA entries.
+static size_t workspace_bytes(int64_t MA, int64_t nnzA) {
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const int64_t ntiles = MA / kScanTile + 2;
+  return al((MA + 1) * 8) * 3 + al(nnzA * 8 + 8) + al(nnzA * 4 + 4) + al(MA * 4) * 3 + al(MA + 1) +
+         al(ntiles * 8);
+}
+
 static int ensure_workspace(Handle *h, int64_t MA, int64_t nnzA) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = MA / kScanTile + 2;
-  size_t need = al((MA + 1) * 8) * 3 + al(nnzA * 8 + 8) + al(nnzA * 4 + 4) + al(MA * 4) * 3 + al(MA + 1) +
-                al(ntiles * 8);
+  const size_t need = workspace_bytes(MA, nnzA);
   if (need > h->ws.bytes) {
     h->ext_rows = -1;  // the cached B row extents go with the old buffer
     if (h->ws.base) NSP_HIP(h, hipFree(h->ws.base));
@@ -1516,6 +1525,109 @@ int graph_destroy(GraphPlan *gp) {
   if (gp->cap_stream) (void)hipStreamDestroy(gp->cap_stream);
   delete gp;
   return NSP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Product without allocation (nsp_spgemm_noalloc): the graph path's kernel
+// sequence (no host synchronisation between kernels; every bin's kernel
+// reads its row count from device memory) run eagerly on the handle's
+// stream, with the per-call counters and the workspace in `scratch`, a
+// file-scope device array, instead of memory the library allocates.  Each
+// device holds its own copy of the array (for the life of the process);
+// its address on a device comes from hipGetSymbolAddress with that device
+// current, looked up once per device (at nsp_create, or at the first call)
+// and kept in g_scratch_ptr.  Host code never passes the array's name to a
+// kernel: it is only a host-side stand-in for the symbol.  Calls on one
+// device share the array: calls that overlap on one device race.
+constexpr size_t kNoAllocScratch = (size_t)NSP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+int noalloc_setup(Handle *h) {
+  if (h->device < 0 || h->device >= kMaxDevices)
+    return set_error(h, NSP_ERR_UNSUPPORTED, "nsp_spgemm_noalloc: device ids past 63 are not supported");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[h->device]) return NSP_OK;
+  int cur = -1;
+  NSP_HIP(h, hipGetDevice(&cur));
+  if (cur != h->device) NSP_HIP(h, hipSetDevice(h->device));
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != h->device) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !p)
+    return set_error(h, NSP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[h->device] = p;
+  return NSP_OK;
+}
+
+template <typename RowT, typename V>
+static int noalloc_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C, int64_t *nnz) {
+  NSP_TRY(noalloc_setup(h));
+  char *base = (char *)g_scratch_ptr[h->device];
+  constexpr size_t kInfoBytes = (sizeof(Info) + 255) & ~(size_t)255;
+  const size_t need = kInfoBytes + workspace_bytes(A->rows, A->nnz);
+  if (need > kNoAllocScratch)
+    return set_error(h, NSP_ERR_UNSUPPORTED,
+                     "nsp_spgemm_noalloc: A needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                         std::to_string(kNoAllocScratch) + " the library holds (use nsp_spgemm)");
+  // a handle on the stack whose counters and workspace are the scratch
+  // array (it owns no device memory and is not destroyed)
+  Handle g;
+  g.device = h->device;
+  g.num_cu = h->num_cu;
+  g.stream = h->stream;
+  g.grid_cap = h->grid_cap;
+  g.sym_small_rows_per_wave = h->sym_small_rows_per_wave;
+  g.sym_rows_per_wave = h->sym_rows_per_wave;
+  g.num_small_rows_per_wave = h->num_small_rows_per_wave;
+  g.num_rows_per_wave = h->num_rows_per_wave;
+  g.heavy_blocks = h->heavy_blocks;
+  g.num_table_factor = h->num_table_factor;
+  g.d_info = reinterpret_cast<Info *>(base);
+  g.ws.base = base + kInfoBytes;
+  g.ws.bytes = kNoAllocScratch - kInfoBytes;  // >= need: ensure_workspace only carves it
+  const int s = graph_record<RowT, V>(&g, A, B, Batch{}, C);
+  if (s != NSP_OK) {
+    h->last_error = g.last_error;
+    return s;
+  }
+  h->have_plan = false;  // the handle's own plan is untouched but no longer current
+  if (!nnz) return NSP_OK;
+  Info hi;
+  NSP_HIP(h, hipMemcpyAsync(&hi, g.d_info, sizeof(Info), hipMemcpyDeviceToHost, h->stream));
+  NSP_HIP(h, hipStreamSynchronize(h->stream));
+  *nnz = hi.nnz;
+  if (hi.error == NSP_ERR_OVERFLOW && sizeof(RowT) == 4 && hi.nnz > (int64_t)INT32_MAX)
+    return set_error(h, NSP_ERR_OVERFLOW, "nsp_spgemm_noalloc: C has 2^31 or more entries; use NSP_IDX64");
+  if (hi.error)
+    return set_error(h, (int)hi.error,
+                     "nsp_spgemm_noalloc: C has " + std::to_string(hi.nnz) + " entries, more than its capacity " +
+                         std::to_string(C->nnz) + " (rows past it were skipped)");
+  return NSP_OK;
+}
+
+int spgemm_noalloc(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                   nsp_csr_t *C, int64_t *nnz) {
+  NSP_TRY(check_csr_args(h, "A", A));
+  NSP_TRY(check_csr_args(h, "B", B));
+  if (A->cols != B->rows) return set_error(h, NSP_ERR_INVALID, "A.cols != B.rows");
+  if (!C || !C->row_ptr || C->nnz < 0 || (C->nnz > 0 && (!C->col || !C->val)))
+    return set_error(h, NSP_ERR_INVALID, "nsp_spgemm_noalloc: C needs a row pointer and col/val of C->nnz entries");
+  if (A->rows < 1) return set_error(h, NSP_ERR_INVALID, "nsp_spgemm_noalloc: A has no rows");
+  if (vt != NSP_F64 && vt != NSP_F32) return set_error(h, NSP_ERR_INVALID, "unknown value type");
+  if (it == NSP_IDX32) {
+    if (A->nnz > INT32_MAX || B->nnz > INT32_MAX || C->nnz > INT32_MAX)
+      return set_error(h, NSP_ERR_OVERFLOW, "nnz >= 2^31 needs NSP_IDX64");
+    return vt == NSP_F64 ? noalloc_impl<int32_t, double>(h, A, B, C, nnz)
+                         : noalloc_impl<int32_t, float>(h, A, B, C, nnz);
+  }
+  if (it == NSP_IDX64)
+    return vt == NSP_F64 ? noalloc_impl<int64_t, double>(h, A, B, C, nnz)
+                         : noalloc_impl<int64_t, float>(h, A, B, C, nnz);
+  return set_error(h, NSP_ERR_INVALID, "unknown index type");
 }
 
 int spgemm_row_extents(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, bool *recomputed) {

@@ -15,6 +15,12 @@ int check_csr(Handle *h, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row);
 int spgemm_batched_symbolic(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const int64_t *a_off,
                             const nsp_csr_t *B, const int64_t *b_off, int64_t count, void *c_row_ptr,
                             int64_t *c_nnz);
+// Product with the workspace in the library's static device array (no
+// allocation); noalloc_setup looks the array's device address up once per
+// device.
+int noalloc_setup(Handle *h);
+int spgemm_noalloc(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                   nsp_csr_t *C, int64_t *nnz);
 struct GraphPlan;
 int graph_create(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const int64_t *a_off,
                  const nsp_csr_t *B, const int64_t *b_off, int64_t count, nsp_csr_t *C, GraphPlan **out);
