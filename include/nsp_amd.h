@@ -102,6 +102,12 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *   "num_rows_per_wave"  rows per wavefront in the numeric bins of 65..256
  *                  entries (1, 2, 4)
  *   "num_table_factor"   numeric hash slots per row entry (2, 4)
+ *   "narrow_mode"  1 (default): when B has at most 4096 columns, rows of up
+ *                  to 1024 products / 512 entries keep their columns in an
+ *                  LDS bitmap (no hash table, no sort); 0: hash tables
+ *   "sym_table_factor"   symbolic hash slots per intermediate product in the
+ *                  bins of up to 256 products (1, 2; default 2: rows whose
+ *                  products are mostly distinct keep the table half empty)
  *   "heavy_blocks" blocks in flight for rows past 16384 products / 4096
  *                  entries (0 = one per CU)
  *   "reuse_extents" 1: nsp_spgemm_backward reuses the per-entry B row

@@ -2,9 +2,9 @@
 
 The library's kernels come in shapes chosen per launch by handle parameters
 (nsp_set_param): rows per wavefront in the small and medium bins of each
-phase (the group size GS = 64 / rows), slots per entry of the numeric hash
-tables, blocks per CU of the grid-stride launches, and blocks in flight for
-heavy rows.  Which shape is fastest depends on the row-length distribution,
+phase (the group size GS = 64 / rows), slots per intermediate product of the
+symbolic hash tables and per row entry of the numeric ones, blocks per CU of
+the grid-stride launches, and blocks in flight for heavy rows.  Which shape is fastest depends on the row-length distribution,
 so the tuner times the real product on the real input:
 
   * the symbolic parameters change only the symbolic pass and the numeric
@@ -30,10 +30,11 @@ import math
 import os
 import statistics
 
-DEFAULTS = {"sym_small_rows_per_wave": 4, "sym_rows_per_wave": 2, "num_small_rows_per_wave": 4,
-            "num_rows_per_wave": 2, "num_table_factor": 2, "grid_cap": 8, "heavy_blocks": 0}
+DEFAULTS = {"sym_small_rows_per_wave": 4, "sym_rows_per_wave": 2, "sym_table_factor": 2,
+            "num_small_rows_per_wave": 4, "num_rows_per_wave": 2, "num_table_factor": 2, "grid_cap": 8,
+            "heavy_blocks": 0}
 
-SPACE_SYM = {"sym_small_rows_per_wave": (1, 2, 4, 8), "sym_rows_per_wave": (1, 2, 4)}
+SPACE_SYM = {"sym_small_rows_per_wave": (1, 2, 4, 8), "sym_rows_per_wave": (1, 2, 4), "sym_table_factor": (1, 2)}
 SPACE_NUM = {"num_small_rows_per_wave": (1, 2, 4, 8), "num_rows_per_wave": (1, 2, 4), "num_table_factor": (2, 4)}
 SPACE_GRID = {"grid_cap": (4, 8, 16)}
 

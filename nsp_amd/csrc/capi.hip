@@ -179,6 +179,15 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
     return rpw(&h->num_small_rows_per_wave, name, true);
   } else if (!std::strcmp(name, "num_rows_per_wave")) {
     return rpw(&h->num_rows_per_wave, name, false);
+  } else if (!std::strcmp(name, "fp32_accumulate")) {
+    if (value != 0 && value != 1) return nsp::set_error(h, NSP_ERR_INVALID, "fp32_accumulate must be 0 or 1");
+    h->fp32_accumulate = (int)value;
+  } else if (!std::strcmp(name, "narrow_mode")) {
+    if (value != 0 && value != 1) return nsp::set_error(h, NSP_ERR_INVALID, "narrow_mode must be 0 or 1");
+    h->narrow_mode = (int)value;
+  } else if (!std::strcmp(name, "sym_table_factor")) {
+    if (value != 1 && value != 2) return nsp::set_error(h, NSP_ERR_INVALID, "sym_table_factor must be 1 or 2");
+    h->sym_table_factor = (int)value;
   } else if (!std::strcmp(name, "num_table_factor")) {
     if (value != 2 && value != 4) return nsp::set_error(h, NSP_ERR_INVALID, "num_table_factor must be 2 or 4");
     h->num_table_factor = (int)value;

@@ -35,6 +35,25 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Tiles of a work list in XCD-contiguous order.  The hardware deals
+// workgroups to the XCDs round-robin (block b runs on XCD b % 8, and each XCD
+// has its own L2), so block b takes tiles from XCD (b % 8)'s contiguous share
+// of the list, every (blocks on that XCD)-th one.  The tiles in flight on one
+// XCD are then neighbours, and so are the B rows they read (neighbouring rows
+// of a stencil or FEM matrix share most of theirs), which keeps those reads
+// in that XCD's L2.  Every tile is visited once for any grid size.
+struct XcdTiles {
+  int64_t first, end, step;
+  __device__ __forceinline__ explicit XcdTiles(int64_t ntiles) {
+    const int nx = gridDim.x < 8 ? (int)gridDim.x : 8;
+    const int x = (int)(blockIdx.x % nx);
+    const int64_t j = blockIdx.x / nx;
+    step = ((int64_t)gridDim.x - x + nx - 1) / nx;  // blocks on this XCD
+    first = ntiles * x / nx + j;
+    end = ntiles * (x + 1) / nx;
+  }
+};
+
 // Multiplicative (Fibonacci) hashing in 24-bit arithmetic: slot = the top LOG
 // bits of (key * floor(2^24/phi)) mod 2^24, with the full-rate 24-bit multiply
 // (v_mul_u32_u24) instead of the quarter-rate 32-bit one and one bit-field

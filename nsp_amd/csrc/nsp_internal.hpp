@@ -74,6 +74,9 @@ struct Handle {
   int64_t ext_rows = -1, ext_nnz = -1;
   int64_t ext_nprod = -1;  // nprod of the cached extents (known once read back)
   int num_table_factor = 2;
+  int sym_table_factor = 2;  // symbolic slots per product bound in bins 0..3 (1 or 2)
+  int narrow_mode = 1;       // 1: column bitmaps instead of hash tables when B has at most 4096 columns
+  int fp32_accumulate = 0;   // 1: FP32 values summed in FP32 where the kernel offers it (narrow products)
   int64_t coo_chunk = int64_t(1) << 21;  // records per streamed chunk of the COO reader
   bool timing = false;
   bool timing_pending = false;

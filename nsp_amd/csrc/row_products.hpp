@@ -164,28 +164,35 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
         for (int u = 0; u < R; u++) prod[u] = VAL ? es->av[W.e[u]] * W.bv[u] : (V)0;
         f(W.key, prod, W.ok, W.bv, W.e);
       };
+      // Inside the loops the loads of the next window are issued with no
+      // branch around them, so that the compiler waits for the current
+      // window's loads alone (vmcnt of one window) instead of every load in
+      // flight; only the last window or two, after the loop, wait for all.
       if constexpr (GS >= 32) {
         // long rows: the two windows alternate roles (no copies)
         Window<R, V> wa, wb;
         load(0, wa);
-        for (int i = 0; i < nwin; i += 2) {
-          if (i + 1 < nwin) load(i + 1, wb);
+        int i = 0;
+        for (; i + 2 < nwin; i += 2) {
+          load(i + 1, wb);
           consume(wa);
-          if (i + 1 < nwin) {
-            if (i + 2 < nwin) load(i + 2, wa);
-            consume(wb);
-          }
+          load(i + 2, wa);
+          consume(wb);
         }
+        if (i + 1 < nwin) load(i + 1, wb);
+        consume(wa);
+        if (i + 1 < nwin) consume(wb);
       } else {
         // short rows (mostly one window): fewer live registers, one more
         // wave per SIMD in the small bins
         Window<R, V> cur, nxt;
         load(0, cur);
-        for (int i = 0; i < nwin; i++) {
-          if (i + 1 < nwin) load(i + 1, nxt);
+        for (int i = 0; i + 1 < nwin; i++) {
+          load(i + 1, nxt);
           consume(cur);
           cur = nxt;
         }
+        consume(cur);
       }
     }
     wave_lds_sync();

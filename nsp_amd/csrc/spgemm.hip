@@ -429,7 +429,9 @@ void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
   bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
   int32_t *K = keys[slot];
-  for (int64_t base = blockIdx.x * (int64_t)RPB; base < count; base += (int64_t)gridDim.x * RPB) {
+  const XcdTiles xt((count + RPB - 1) / RPB);
+  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
+    const int64_t base = tile * RPB;
     const int64_t t = base + slot;
     if (t < count) {
       const int32_t r = perm ? perm[off + t] : (int32_t)t;
@@ -598,7 +600,9 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
   int32_t *K = keys[slot];
   Acc *W = vals[slot];
   uint32_t *S = stage[slot];
-  for (int64_t base = blockIdx.x * (int64_t)RPB; base < count; base += (int64_t)gridDim.x * RPB) {
+  const XcdTiles xt((count + RPB - 1) / RPB);
+  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
+    const int64_t base = tile * RPB;
     const int64_t t = base + slot;
     if (t < count) {
       const int32_t r = perm ? perm[off + t] : (int32_t)t;
@@ -659,6 +663,197 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
           }
         }
       }
+      wave_lds_sync();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Narrow products (B with at most kNarrowCols columns, such as a batch of
+// matrices of up to 4096 rows and columns stored as one concatenated CSR):
+// a row's columns as an LDS bitmap of kNarrowCols bits instead of a hash
+// table.  The symbolic pass sets one bit per product (an atomic OR: no
+// compare-and-swap and no probing, however many of the products are
+// distinct) and counts the bits.  The numeric pass sets the bits, turns
+// them into each column's position in the sorted row (a running count of
+// set bits per word), walks the products a second time adding each into
+// the value slot of its position, and writes the row in column order: no
+// sort.  Columns are masked to the bitmap (a column past it, which
+// violates the product's preconditions, cannot write outside the row's
+// bitmap).
+constexpr int kNarrowCols = 4096;
+constexpr int kNarrowWords = kNarrowCols / 32;
+
+template <int GS, int R, bool O32>
+__global__ __launch_bounds__(256) void k_sym_narrow(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                    const Info *__restrict__ info,
+                                                    const int32_t *__restrict__ perm,
+                                                    const int64_t *__restrict__ arpt,
+                                                    const int64_t *__restrict__ ab_b0,
+                                                    const int32_t *__restrict__ ab_len,
+                                                    const int32_t *__restrict__ bcol,
+                                                    int32_t *__restrict__ row_nz) {
+  constexpr int RPB = 256 / GS;
+  constexpr int WPL = kNarrowWords / GS;  // bitmap words per lane in the count
+  __shared__ __align__(16) uint32_t bits[RPB][kNarrowWords];
+  __shared__ EntryScratch<GS, float> ent[RPB];
+  int64_t count, off;
+  bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
+  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
+  uint32_t *Bm = bits[slot];
+  const XcdTiles xt((count + RPB - 1) / RPB);
+  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
+    const int64_t t = tile * RPB + slot;
+    if (t < count) {
+      const int32_t r = perm ? perm[off + t] : (int32_t)t;
+      clear_lds<GS, kNarrowWords>(Bm, glane, 0);
+      wave_lds_sync();
+      row_products<GS, R, false, float, false, O32>(
+          glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[slot],
+          [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              // lanes past the end OR nothing into a word of their own
+              const uint32_t k = (uint32_t)key[u] & (kNarrowCols - 1);
+              atomicOr(&Bm[ok[u] ? (k >> 5) : (uint32_t)glane], ok[u] ? (1u << (k & 31)) : 0u);
+            }
+          });
+      wave_lds_sync();
+      int cnt = 0;
+      if constexpr (WPL >= 4) {
+        const uint4 *B4 = reinterpret_cast<const uint4 *>(Bm + glane * WPL);
+#pragma unroll
+        for (int q = 0; q < WPL / 4; q++) {
+          const uint4 x = B4[q];
+          cnt += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < WPL; q++) cnt += __popc(Bm[glane * WPL + q]);
+      }
+      cnt = group_sum<GS>(cnt);
+      if (glane == 0) row_nz[r] = cnt;
+      wave_lds_sync();
+    }
+  }
+}
+
+template <typename V, int GS, int NS, int R, bool O32, typename AccN = Acc>
+__global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                    const Info *__restrict__ info,
+                                                    const int32_t *__restrict__ perm,
+                                                    const int64_t *__restrict__ arpt,
+                                                    const int64_t *__restrict__ ab_b0,
+                                                    const V *__restrict__ aval,
+                                                    const int32_t *__restrict__ ab_len,
+                                                    const int32_t *__restrict__ bcol,
+                                                    const V *__restrict__ bval,
+                                                    const int64_t *__restrict__ crpt,
+                                                    int32_t *__restrict__ ccol,
+                                                    V *__restrict__ cval) {
+  constexpr int RPB = 256 / GS;
+  constexpr int WPL = kNarrowWords / GS;
+  // per bitmap word: .x its bits, .y the set bits of the row's words before it
+  __shared__ __align__(16) uint2 wb[RPB][kNarrowWords];
+  __shared__ __align__(16) AccN vals[RPB][NS + GS];  // + one dummy slot per lane
+  __shared__ EntryScratch<GS, V> ent[RPB];
+  int64_t count, off;
+  bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
+  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
+  uint2 *Wb = wb[slot];
+  AccN *Vs = vals[slot];
+  // adds a window's products into the value slots of their positions
+  auto accumulate = [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
+    bool live[R];
+#pragma unroll
+    for (int u = 0; u < R; u++) live[u] = __ballot(ok[u]) != 0;
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+      if (!live[u]) continue;
+      const uint32_t k = (uint32_t)key[u] & (kNarrowCols - 1);
+      const uint2 e = Wb[k >> 5];
+      const uint32_t rank = e.y + (uint32_t)__popc(e.x & ((1u << (k & 31)) - 1u));
+      const uint32_t idx = ok[u] ? rank : (uint32_t)(NS + glane);
+      __hip_atomic_fetch_add(&Vs[idx], (AccN)prod[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  };
+  const XcdTiles xt((count + RPB - 1) / RPB);
+  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
+    const int64_t t = tile * RPB + slot;
+    if (t < count) {
+      const int32_t r = perm ? perm[off + t] : (int32_t)t;
+      if (crpt[r + 1] > info->cap) {  // C buffers too small: skip the row, flag it
+        if (glane == 0) atomicOr((unsigned int *)&const_cast<Info *>(info)->error, (unsigned int)NSP_ERR_OVERFLOW);
+        continue;
+      }
+      clear_lds<GS, kNarrowWords>(Wb, glane, 0);
+      clear_lds<GS, NS + GS>(Vs, glane, 0);
+      wave_lds_sync();
+      const int64_t a0 = arpt[r], a1 = arpt[r + 1];
+      // walk 1: the row's columns into the bitmap; a row that is one window
+      // long keeps its products in registers for the accumulation
+      int32_t sk[R];
+      V sp[R];
+      bool so[R];
+      int windows = 0;
+      row_products<GS, R, true, V, true, O32>(
+          glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
+          [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
+#pragma unroll
+            for (int u = 0; u < R; u++) {
+              // lanes past the end OR nothing into a word of their own
+              const uint32_t k = (uint32_t)key[u] & (kNarrowCols - 1);
+              atomicOr(&Wb[ok[u] ? (k >> 5) : (uint32_t)glane].x, ok[u] ? (1u << (k & 31)) : 0u);
+              sk[u] = key[u];
+              sp[u] = prod[u];
+              so[u] = ok[u];
+            }
+            windows++;
+          });
+      wave_lds_sync();
+      // running counts: lane g holds words [g*WPL, (g+1)*WPL)
+      {
+        uint32_t cw[WPL];
+        int c = 0;
+#pragma unroll
+        for (int w = 0; w < WPL; w++) {
+          cw[w] = (uint32_t)__popc(Wb[glane * WPL + w].x);
+          c += (int)cw[w];
+        }
+        uint32_t run = (uint32_t)(group_incl_scan<GS>(c, glane) - c);
+#pragma unroll
+        for (int w = 0; w < WPL; w++) {
+          Wb[glane * WPL + w].y = run;
+          run += cw[w];
+        }
+      }
+      wave_lds_sync();
+      if (windows == 1) {
+        accumulate(sk, sp, so);
+      } else {
+        // walk 2: each product into the slot of its column's position
+        row_products<GS, R, true, V, true, O32>(
+            glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
+            [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
+              accumulate(key, prod, ok);
+            });
+      }
+      wave_lds_sync();
+      // the row in column order: columns from the bits, values in order
+      const int64_t c0 = crpt[r];
+      const int nz = (int)(crpt[r + 1] - c0);
+#pragma unroll
+      for (int w = 0; w < WPL; w++) {
+        const uint2 e = Wb[glane * WPL + w];
+        uint32_t x = e.x, rk = e.y;
+        while (x) {
+          const int b = __ffs(x) - 1;
+          x &= x - 1u;
+          if ((int)rk < nz) ccol[c0 + rk] = (glane * WPL + w) * 32 + b;
+          rk++;
+        }
+      }
+      for (int j = glane; j < nz; j += GS) cval[c0 + j] = (V)Vs[j];
       wave_lds_sync();
     }
   }
@@ -964,6 +1159,8 @@ constexpr int kSymR = 8;  // B loads in flight per lane, symbolic (at most)
 // exceeds the bin's product bound SH (tables hold SH >= the row's products),
 // so the lanes of a short row do not issue dummy inserts; at least 2.
 constexpr int sym_r(int GS, int SH) { return SH / GS >= kSymR ? kSymR : (SH / GS >= 4 ? 4 : 2); }
+// Symbolic table slots of a bin of product bound PB with the doubled table.
+constexpr int sym_slots2(int PB) { return PB <= 256 ? 2 * PB : PB; }
 constexpr int kNumR = 4;  // B loads in flight per lane, numeric (at most)
 
 // Loads per lane of a numeric group kernel: a window of GS*R products is
@@ -977,22 +1174,120 @@ constexpr int num_r(int GS, int NS) { return NS / GS >= kNumR ? kNumR : 2; }
 // values: at most 2^29 entries).
 static bool offsets32(int64_t nnzB) { return nnzB <= ((int64_t)1 << 29); }
 
+// Narrow kernels of the wave-level bins 0..5, with the group size of the
+// bin's rows-per-wave parameter.
+template <int PB>
+static void launch_sym_narrow(Handle *h, int rpw, bool o32, int lo, int bin, int64_t direct, int64_t count,
+                              const int32_t *perm, const int64_t *arpt, const int64_t *ab_b0, const int32_t *ab_len,
+                              const int32_t *bcol) {
+#define SYM_NARROW(GS)                                                                                       \
+  do {                                                                                                       \
+    const dim3 grid(grid_for(h, count, 256 / GS, h->grid_cap));                                              \
+    if (o32)                                                                                                 \
+      hipLaunchKernelGGL((k_sym_narrow<GS, sym_r(GS, PB), true>), grid, dim3(256), 0, h->stream, lo, bin, direct, \
+                         h->d_info, perm, arpt, ab_b0, ab_len, bcol, h->row_nz);                             \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_sym_narrow<GS, sym_r(GS, PB), false>), grid, dim3(256), 0, h->stream, lo, bin,   \
+                         direct, h->d_info, perm, arpt, ab_b0, ab_len, bcol, h->row_nz);                     \
+  } while (0)
+  if (rpw >= 8 && PB <= 128) SYM_NARROW(8);
+  else if (rpw >= 4) SYM_NARROW(16);
+  else if (rpw == 2) SYM_NARROW(32);
+  else SYM_NARROW(64);
+#undef SYM_NARROW
+}
+
+template <typename V, int GS, int NS>
+static void launch_num_narrow_gs(Handle *h, bool o32, int lo, int bin, int64_t direct, int64_t count,
+                                 const int32_t *perm, const int64_t *arpt, const int64_t *ab_b0, const V *aval,
+                                 const int32_t *ab_len, const int32_t *bcol, const V *bval, int32_t *ccol, V *cval) {
+  const dim3 grid(grid_for(h, count, 256 / GS, h->grid_cap));
+  if constexpr (sizeof(V) == 4) {
+    if (h->fp32_accumulate) {  // FP32 values summed in FP32
+      if (o32)
+        hipLaunchKernelGGL((k_num_narrow<V, GS, NS, num_r(GS, NS), true, float>), grid, dim3(256), 0, h->stream, lo,
+                           bin, direct, h->d_info, perm, arpt, ab_b0, aval, ab_len, bcol, bval, h->c_rpt64, ccol, cval);
+      else
+        hipLaunchKernelGGL((k_num_narrow<V, GS, NS, num_r(GS, NS), false, float>), grid, dim3(256), 0, h->stream,
+                           lo, bin, direct, h->d_info, perm, arpt, ab_b0, aval, ab_len, bcol, bval, h->c_rpt64, ccol,
+                           cval);
+      return;
+    }
+  }
+  if (o32)
+    hipLaunchKernelGGL((k_num_narrow<V, GS, NS, num_r(GS, NS), true>), grid, dim3(256), 0, h->stream, lo, bin,
+                       direct, h->d_info, perm, arpt, ab_b0, aval, ab_len, bcol, bval, h->c_rpt64, ccol, cval);
+  else
+    hipLaunchKernelGGL((k_num_narrow<V, GS, NS, num_r(GS, NS), false>), grid, dim3(256), 0, h->stream, lo, bin,
+                       direct, h->d_info, perm, arpt, ab_b0, aval, ab_len, bcol, bval, h->c_rpt64, ccol, cval);
+}
+
+// Group size from the bin's rows-per-wave parameter, within what the bin's
+// value slots allow in LDS (8 lanes for rows of up to 64 entries, 16 up to
+// 256) and never more lanes than its largest row has entries.
+template <typename V, int NS>
+static void launch_num_narrow(Handle *h, int rpw, bool o32, int lo, int bin, int64_t direct, int64_t count,
+                              const int32_t *perm, const int64_t *arpt, const int64_t *ab_b0, const V *aval,
+                              const int32_t *ab_len, const int32_t *bcol, const V *bval, int32_t *ccol, V *cval) {
+  int gs = rpw >= 8 ? 8 : rpw >= 4 ? 16 : rpw == 2 ? 32 : 64;
+  if (NS > 64 && gs < 16) gs = 16;
+  if (NS > 256 && gs < 32) gs = 32;
+  if (gs > NS) gs = NS < 8 ? 8 : NS;
+#define NUM_NARROW(GS) \
+  launch_num_narrow_gs<V, GS, NS>(h, o32, lo, bin, direct, count, perm, arpt, ab_b0, aval, ab_len, bcol, bval, ccol, cval)
+  if constexpr (NS <= 64) {
+    if (gs == 8) { NUM_NARROW(8); return; }
+  }
+  if constexpr (NS <= 256) {
+    if (gs == 16) { NUM_NARROW(16); return; }
+  }
+  if constexpr (NS >= 32) {
+    if (gs == 32) { NUM_NARROW(32); return; }
+  }
+  if constexpr (NS >= 64) {
+    NUM_NARROW(64);
+  }
+#undef NUM_NARROW
+}
+
 static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
                            const int64_t *arpt, const int64_t *ab_b0, const int32_t *ab_len,
-                           const int32_t *bcol, int64_t nnzB) {
+                           const int32_t *bcol, int64_t nnzB, int64_t ncols) {
+  if (h->narrow_mode && ncols <= kNarrowCols && bin <= 5) {
+    const int rpw = bin <= 2 ? h->sym_small_rows_per_wave : h->sym_rows_per_wave;
+    const bool o32 = offsets32(nnzB);
+    const int32_t *perm = direct >= 0 ? nullptr : h->perm_sym;
+    switch (bin) {
+      case 0: launch_sym_narrow<32>(h, rpw, o32, lo, bin, direct, count, perm, arpt, ab_b0, ab_len, bcol); break;
+      case 1: launch_sym_narrow<64>(h, rpw, o32, lo, bin, direct, count, perm, arpt, ab_b0, ab_len, bcol); break;
+      case 2: launch_sym_narrow<128>(h, rpw, o32, lo, bin, direct, count, perm, arpt, ab_b0, ab_len, bcol); break;
+      case 3: launch_sym_narrow<256>(h, rpw, o32, lo, bin, direct, count, perm, arpt, ab_b0, ab_len, bcol); break;
+      case 4: launch_sym_narrow<512>(h, rpw, o32, lo, bin, direct, count, perm, arpt, ab_b0, ab_len, bcol); break;
+      default: launch_sym_narrow<1024>(h, rpw, o32, lo, bin, direct, count, perm, arpt, ab_b0, ab_len, bcol); break;
+    }
+    return;
+  }
   hipStream_t st = h->stream;
   const Info *info = h->d_info;
   const int32_t *perm = direct >= 0 ? nullptr : h->perm_sym;
   int32_t *rnz = h->row_nz;
   const int cap = h->grid_cap;
   const bool o32 = offsets32(nnzB);
-#define SYM_GROUP(GS, SH)                                                                          \
+  // table slots: the bin's product bound PB, or twice that in bins 0..3
+  // ("sym_table_factor" 2): a row whose products are mostly distinct fills
+  // a table of PB slots almost completely, and linear probing then walks far
+#define SYM_GROUP_T(GS, PB, SH)                                                                    \
   if (o32)                                                                                         \
-    hipLaunchKernelGGL((k_sym_group<GS, SH, sym_r(GS, SH), true>), dim3(grid_for(h, count, 256 / GS, cap)), \
+    hipLaunchKernelGGL((k_sym_group<GS, SH, sym_r(GS, PB), true>), dim3(grid_for(h, count, 256 / GS, cap)), \
                        dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz); \
   else                                                                                             \
-    hipLaunchKernelGGL((k_sym_group<GS, SH, sym_r(GS, SH), false>), dim3(grid_for(h, count, 256 / GS, cap)), \
+    hipLaunchKernelGGL((k_sym_group<GS, SH, sym_r(GS, PB), false>), dim3(grid_for(h, count, 256 / GS, cap)), \
                        dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
+#define SYM_GROUP(GS, PB)                                                                          \
+  do {                                                                                             \
+    if (PB <= 256 && h->sym_table_factor >= 2) { SYM_GROUP_T(GS, PB, sym_slots2(PB)); }           \
+    else { SYM_GROUP_T(GS, PB, PB); }                                                              \
+  } while (0)
 #define SYM_BLOCK(BS, SH)                                                                          \
   hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, cap)), dim3(BS), 0, \
                      st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
@@ -1023,6 +1318,7 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
 #undef SYM_RPW_SMALL
 #undef SYM_RPW
 #undef SYM_GROUP
+#undef SYM_GROUP_T
 #undef SYM_BLOCK
 }
 
@@ -1039,6 +1335,23 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
                            const int64_t *arpt, const int64_t *ab_b0, const V *aval,
                            const int32_t *ab_len, const int32_t *bcol, const V *bval,
                            int32_t *ccol, V *cval, int64_t ncols, int64_t nnzB) {
+  if (h->narrow_mode && ncols <= kNarrowCols && bin <= 5) {
+    const int rpw = bin <= 2 ? h->num_small_rows_per_wave : h->num_rows_per_wave;
+    const bool o32 = offsets32(nnzB);
+    const int32_t *perm = direct >= 0 ? nullptr : h->perm_num;
+#define NN(NS) \
+  launch_num_narrow<V, NS>(h, rpw, o32, lo, bin, direct, count, perm, arpt, ab_b0, aval, ab_len, bcol, bval, ccol, cval)
+    switch (bin) {
+      case 0: NN(16); break;
+      case 1: NN(32); break;
+      case 2: NN(64); break;
+      case 3: NN(128); break;
+      case 4: NN(256); break;
+      default: NN(512); break;
+    }
+#undef NN
+    return;
+  }
   hipStream_t st = h->stream;
   const Info *info = h->d_info;
   const int32_t *perm = direct >= 0 ? nullptr : h->perm_num;
@@ -1277,7 +1590,7 @@ static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void
     timing_mark(h, 1);
     for (int q = 0; q < n; q++)
       launch_sym_bin(h, L[q].bin, L[q].lo, L[q].direct, L[q].count, h->a_rpt64, h->ab_b0, h->ab_len, B->col,
-                     B->nnz);
+                     B->nnz, B->cols);
     h->stats.sym_launches = n;
     if (n == 1 && L[0].direct >= 0) h->stats.direct_mode |= 1;
   }
@@ -1412,7 +1725,7 @@ static int graph_record(Handle *g, const nsp_csr_t *A, const nsp_csr_t *B, const
   NSP_TRY(run_rowinfo<RowT>(g, A, B, bt, C->nnz));
   launch_scatter(g, MA, g->d_info->count_sym, g->d_info->cursor_sym, g->perm_sym);
   for (int b = 0; b < kBins; b++)
-    launch_sym_bin(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, g->ab_len, B->col, B->nnz);
+    launch_sym_bin(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, g->ab_len, B->col, B->nnz, B->cols);
   hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)(MA / kScanTile + 1)), dim3(kScanBS), 0, g->stream, MA, g->row_nz,
                      (RowT *)C->row_ptr, g->c_rpt64, g->cls, g->tile_status, g->d_info);
   launch_scatter(g, MA, g->d_info->count_num, g->d_info->cursor_num, g->perm_num);
@@ -1447,6 +1760,9 @@ int graph_create(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, 
   g->num_rows_per_wave = h->num_rows_per_wave;
   g->heavy_blocks = h->heavy_blocks;
   g->num_table_factor = h->num_table_factor;
+  g->sym_table_factor = h->sym_table_factor;
+  g->narrow_mode = h->narrow_mode;
+  g->fp32_accumulate = h->fp32_accumulate;
   gp->rows = A->rows;
   gp->launch_stream = h->stream;
   auto fail = [&](int s) {
@@ -1586,6 +1902,9 @@ static int noalloc_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
   g.num_rows_per_wave = h->num_rows_per_wave;
   g.heavy_blocks = h->heavy_blocks;
   g.num_table_factor = h->num_table_factor;
+  g.sym_table_factor = h->sym_table_factor;
+  g.narrow_mode = h->narrow_mode;
+  g.fp32_accumulate = h->fp32_accumulate;
   g.d_info = reinterpret_cast<Info *>(base);
   g.ws.base = base + kInfoBytes;
   g.ws.bytes = kNoAllocScratch - kInfoBytes;  // >= need: ensure_workspace only carves it

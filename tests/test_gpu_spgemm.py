@@ -66,37 +66,49 @@ def mixed_rows(nrows, ncols, lo, hi, seed):
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_all_bins(nsp, orc, dtype):
     """Row lengths spread so that every symbolic bin up to 16384 products and
-    every numeric bin up to 4096 entries is populated."""
+    every numeric bin up to 4096 entries is populated; B has 4000 columns, so
+    the wave-level bins run the column-bitmap kernels by default and the
+    hash-table kernels with narrow_mode 0."""
     print("case: all bins", dtype, flush=True)
     A = to_dtype(mixed_rows(3000, 2500, 1, 150, seed=1), dtype)
     B = to_dtype(mixed_rows(2500, 4000, 1, 100, seed=2), dtype)
-    C, h = run(nsp, A, B, dtype)
-    st = h.stats()
-    assert all(c > 0 for c in st["sym_bin"][:8]), st["sym_bin"]
-    assert all(c > 0 for c in st["num_bin"][:8]), st["num_bin"]
-    check_against_oracle(A, B, C, dtype, orc)
+    ref = orc.spgemm(A, B)
+    for narrow in (1, 0):
+        h = nsp.Handle()
+        h.set_param("narrow_mode", narrow)
+        C, h = run(nsp, A, B, dtype, handle=h)
+        st = h.stats()
+        assert all(c > 0 for c in st["sym_bin"][:8]), st["sym_bin"]
+        assert all(c > 0 for c in st["num_bin"][:8]), st["num_bin"]
+        check_against_oracle(A, B, C, dtype, orc, ref=ref)
 
 
 @pytest.mark.parametrize("rpw", [1, 2, 4, 8])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_rows_per_wave_and_table_variants(nsp, orc, rpw, dtype):
     """Every wave-level kernel shape (1, 2 or 4 rows per wavefront in each
-    bin group, 8 in the small bins, table factor 2 and 4) gives the same
-    product."""
+    bin group, 8 in the small bins; column bitmaps, and hash tables of both
+    symbolic and both numeric table factors) gives the same product."""
     print("case: rows per wave", rpw, dtype, flush=True)
     A = to_dtype(mixed_rows(1500, 1200, 1, 60, seed=3), dtype)
     B = to_dtype(mixed_rows(1200, 1600, 1, 40, seed=4), dtype)
-    for tf in (2, 4):
+    ref = orc.spgemm(A, B)
+    # (narrow_mode, sym_table_factor, num_table_factor): the column-bitmap
+    # kernels (B has 1600 columns), then the hash kernels with both symbolic
+    # and both numeric table sizes
+    for narrow, stf, ntf in ((1, 2, 2), (0, 1, 2), (0, 2, 4)):
         h = nsp.Handle()
         for name in ("sym_small_rows_per_wave", "sym_rows_per_wave", "num_small_rows_per_wave",
                      "num_rows_per_wave"):
             h.set_param(name, min(rpw, 4) if name in ("sym_rows_per_wave", "num_rows_per_wave") else rpw)
-        h.set_param("num_table_factor", tf)
+        h.set_param("narrow_mode", narrow)
+        h.set_param("sym_table_factor", stf)
+        h.set_param("num_table_factor", ntf)
         C, _ = run(nsp, A, B, dtype, handle=h)
         st = h.stats()
         assert all(c > 0 for c in st["sym_bin"][:6]), st["sym_bin"]
         assert all(c > 0 for c in st["num_bin"][:5]), st["num_bin"]
-        check_against_oracle(A, B, C, dtype, orc)
+        check_against_oracle(A, B, C, dtype, orc, ref=ref)
 
 
 def test_structural_zeros_and_empty_rows(nsp, orc):
