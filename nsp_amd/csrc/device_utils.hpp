@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace nsp {
 
 constexpr int kWave = 64;
@@ -165,11 +167,39 @@ __device__ __forceinline__ void hash_accumulate_batch(int32_t *keys, V *vals, co
 }
 
 // ---- group (sub-wave) collectives; GS lanes, GS a power of two <= 64 ------
+// The 32-bit value of the group's last lane in every lane of the group,
+// without an LDS round trip: DPP row_share (row_newbcast) for groups within
+// a 16-lane row, v_readlane for groups of 32 and 64.
+template <int GS>
+__device__ __forceinline__ int group_last(int x) {
+  if constexpr (GS == 8) {
+    const int lo = __builtin_amdgcn_update_dpp(0, x, 0x157, 0xF, 0xF, false);  // lane 7 of the row
+    const int hi = __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xF, 0xF, false);  // lane 15 of the row
+    return (__lane_id() & 8) ? hi : lo;
+  } else if constexpr (GS == 16) {
+    return __builtin_amdgcn_update_dpp(0, x, 0x15F, 0xF, 0xF, false);
+  } else if constexpr (GS == 32) {
+    const int a = __builtin_amdgcn_readlane(x, 31), b = __builtin_amdgcn_readlane(x, 63);
+    return __lane_id() < 32 ? a : b;
+  } else {
+    return __builtin_amdgcn_readlane(x, 63);
+  }
+}
+
+template <int GS, int RL = 0>
+__device__ __forceinline__ int group_incl_scan_dpp(int v, int glane);
+
 template <int GS, typename T>
 __device__ __forceinline__ T group_sum(T x) {
+  if constexpr (sizeof(T) == 4 && std::is_integral<T>::value && GS >= 8) {
+    // an inclusive scan (DPP) and the last lane's total
+    const int glane = (int)(__lane_id() % GS);
+    return (T)group_last<GS>(group_incl_scan_dpp<GS>((int)x, glane));
+  } else {
 #pragma unroll
-  for (int o = GS / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GS);
-  return x;
+    for (int o = GS / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GS);
+    return x;
+  }
 }
 
 template <int GS, typename T>
@@ -182,15 +212,36 @@ __device__ __forceinline__ T group_max(T x) {
   return x;
 }
 
-// inclusive scan within groups of GS lanes
+// inclusive scan within groups of GS lanes (GS >= 2).  32-bit integers use DPP lane
+// moves (VALU, no LDS round trip): row_shr:1,2,4,8 within each 16-lane row,
+// then row_bcast:15 (lane 15 of a row added into the next row, rows 1 and 3)
+// and row_bcast:31 (lane 31 into rows 2 and 3) to join rows into groups of 32
+// and 64.  Other types go through shuffles.
+template <int GS, int RL>
+__device__ __forceinline__ int group_incl_scan_dpp(int v, int glane) {
+  // rows of 16 lanes hold several groups when GS < 16
+  auto step = [&](int y, int o) { v += (GS < 16 && glane < o) ? 0 : y; };
+  step(__builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false), 1);  // row_shr:1
+  if constexpr (GS > 2) step(__builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false), 2);
+  if constexpr (GS > 4) step(__builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false), 4);
+  if constexpr (GS > 8) step(__builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false), 8);
+  if constexpr (GS >= 32) v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  if constexpr (GS == 64) v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return v;
+}
+
 template <int GS, typename T>
 __device__ __forceinline__ T group_incl_scan(T x, int glane) {
+  if constexpr (sizeof(T) == 4 && std::is_integral<T>::value) {
+    return (T)group_incl_scan_dpp<GS>((int)x, glane);
+  } else {
 #pragma unroll
-  for (int o = 1; o < GS; o <<= 1) {
-    T y = __shfl_up(x, o, GS);
-    if (glane >= o) x += y;
+    for (int o = 1; o < GS; o <<= 1) {
+      T y = __shfl_up(x, o, GS);
+      if (glane >= o) x += y;
+    }
+    return x;
   }
-  return x;
 }
 
 // ---- LDS hash tables -------------------------------------------------------

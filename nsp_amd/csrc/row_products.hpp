@@ -89,7 +89,7 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
     const uint64_t nzm = group_ballot<GS>(len > 0);
     const int end = group_incl_scan<GS>(len, glane);
     const int start = end - len;
-    const int T = __shfl(end, GS - 1, GS);
+    const int T = group_last<GS>(end);
     if (len > 0) {
       const int cidx = __popcll(nzm & lt);
       es->base[cidx] = b0 - start;
