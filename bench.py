@@ -57,16 +57,30 @@ def build_c1():
 WORKLOADS = {"c2": build_c2, "c1": build_c1}
 
 
-def algorithmic_bytes(A, B, nnzC, s=8):
+def algorithmic_bytes(A, B, nnzC, s=8, ip=4, count=1):
+    """Unique HBM bytes of the two-pass product (symbolic, scan, numeric):
+    each array read or written once; s = value width, ip = row-pointer
+    width, count = matrices in a concatenated batch (one row-pointer end
+    each)."""
     nA, nB = A.nrows, B.nrows
-    sym = 4 * ((nA + 1) + A.nnz + (nB + 1) + B.nnz) + 4 * nA
-    scan = 4 * nA + 4 * (nA + 1)
-    num = 4 * (nA + 1) + (4 + s) * A.nnz + 4 * (nB + 1) + (4 + s) * B.nnz + 4 * (nA + 1) + (4 + s) * nnzC
+    rA, rB = ip * (nA + count), ip * (nB + count)
+    sym = rA + 4 * A.nnz + rB + 4 * B.nnz + 4 * nA
+    scan = 4 * nA + ip * (nA + count)
+    num = rA + (4 + s) * A.nnz + rB + (4 + s) * B.nnz + ip * (nA + count) + (4 + s) * nnzC
     return sym, scan, num
 
 
+def roofline_hbm(alg_bytes, ms, kernel, traffic=None):
+    achieved = alg_bytes / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": int(alg_bytes),
+            "kernel_ms": round(ms, 4)}
+
+
 TRAFFIC_FILES = {"c2": "spgemm_numeric_stencil128_rank0_traffic.json",
-                 "coo": "coo_read_stencil128_rank0_traffic.json"}
+                 "coo": "coo_read_stencil128_rank0_traffic.json",
+                 "batched": "spgemm_numeric_batched4096_traffic.json",
+                 "rmat": "spgemm_numeric_rmat20_traffic.json"}
 
 
 def measured_traffic(workload: str):
@@ -201,11 +215,14 @@ def fwd_bwd_leg(dtype: str, m: int = 64, steps: int = 50, warmup: int = 5):
                     "gather, dA and dB (2 host syncs)"}
 
 
-def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int = 2):
+def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int = 2, params=None,
+                loop: bool = True):
     """`count` independent FP32 products C_i = A_i*A_i of uneven sizes: one
-    hipGraph launch over the concatenated batch vs a per-matrix loop of the
+    hipGraph launch over the concatenated batch vs the eager batched call
+    (same kernels, two host synchronisations) and a per-matrix loop of the
     plain two-phase product on the same GPU; results compared entry by
-    entry."""
+    entry.  `params` fixes the launch parameters instead of autotuning them
+    (profiling runs); `loop` False skips the per-matrix loop."""
     import torch
     import nsp_amd
     from nsp_amd import gen
@@ -214,13 +231,42 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
     d_off = torch.from_numpy(off).cuda()
     h = nsp_amd.Handle()
     from nsp_amd import autotune
-    tuned = autotune.tune(h, dX, dX, d_off, d_off, reps=2, use_cache=False)
+    if params:
+        autotune.apply(h, params)
+        tuned = dict(params)
+    else:
+        tuned = autotune.tune(h, dX, dX, d_off, d_off, reps=2, use_cache=False)
     _, nnz = h.batched_symbolic(dX, d_off, dX, d_off)
     nprod = h.stats()["nprod"]
     g = h.graph(dX, dX, capacity=nnz, a_off=d_off, b_off=d_off)
     t_graph = _timed(g.launch, steps, warmup)
     Cg = g.result()
-    t_eager = _timed(lambda: h.batched_spgemm(dX, d_off, dX, d_off), 2, 1)
+    t_eager = _timed(lambda: h.batched_spgemm(dX, d_off, dX, d_off), steps, warmup)
+    # phase times of the eager call (HIP events on the handle's stream)
+    h.set_param("timing", 1)
+    ph = {"t_bin_ms": [], "t_sym_ms": [], "t_scan_ms": [], "t_num_ms": []}
+    for _ in range(3):
+        h.batched_spgemm(dX, d_off, dX, d_off)
+        st = h.stats()
+        for k in ph:
+            ph[k].append(st[k])
+    h.set_param("timing", 0)
+    ph = {k: float(np.median(v)) for k, v in ph.items()}
+    sym_b, scan_b, num_b = algorithmic_bytes(X, X, nnz, s=4, count=count)
+    out = {"workload": f"{count} uneven FP32 products A_i*A_i, n_i~U[64,4096], Poisson(8) entries/row",
+           "rows": X.nrows, "nnz": X.nnz, "nnz_C": nnz, "nprod": nprod,
+           "accumulate": "f64 (FP32 products summed in FP64 LDS slots, rounded once)",
+           "graph_ms": round(t_graph, 3), "eager_batched_ms": round(t_eager, 3),
+           "graph_speedup_vs_eager": round(t_eager / t_graph, 3),
+           "graph_GFLOPs": round(2.0 * nprod / t_graph / 1e6, 1), "autotuned": tuned,
+           "phases_ms_eager": {"binning": round(ph["t_bin_ms"], 3), "symbolic": round(ph["t_sym_ms"], 3),
+                               "scan": round(ph["t_scan_ms"], 3), "numeric": round(ph["t_num_ms"], 3)},
+           "roofline": roofline_hbm(num_b, ph["t_num_ms"], "numeric pass (column-bitmap kernels), eager",
+                                    measured_traffic("batched")),
+           "two_pass_bytes": sym_b + scan_b + num_b,
+           "two_pass_frac_graph": round((sym_b + scan_b + num_b) / (t_graph / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if not loop:
+        return out
 
     mats = []
     for b in range(count):
@@ -251,12 +297,9 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
     absC = g.result().val.clone()
     dX.val.copy_(saved)
     err = float(((vg - val).abs() / absC.clamp_min(1e-30)).max()) if same and nnz else None
-    return {"workload": f"{count} uneven FP32 products A_i*A_i, n_i~U[64,4096], Poisson(8) entries/row",
-            "rows": X.nrows, "nnz": X.nnz, "nnz_C": nnz, "nprod": nprod,
-            "graph_ms": round(t_graph, 3), "eager_batched_ms": round(t_eager, 3),
-            "loop_ms": round(t_loop, 3), "speedup_vs_loop": round(t_loop / t_graph, 2),
-            "graph_GFLOPs": round(2.0 * nprod / t_graph / 1e6, 1), "autotuned": tuned,
-            "structure_equal": bool(same), "max_componentwise_diff": err}
+    out.update({"loop_ms": round(t_loop, 3), "speedup_vs_loop": round(t_loop / t_graph, 2),
+                "structure_equal": bool(same), "max_componentwise_diff": err})
+    return out
 
 
 def rmat_leg(scale: int = 20, steps: int = 3, warmup: int = 1):
@@ -279,7 +322,12 @@ def rmat_leg(scale: int = 20, steps: int = 3, warmup: int = 1):
 
     ms = _timed(step, steps, warmup)
     st2 = h.stats()
+    sym_b, scan_b, num_b = algorithmic_bytes(A, A, nnz, s=8, ip=8)
     out = {"workload": f"R-MAT scale {scale} (a,b,c)=(.57,.19,.19), 16 edges/vertex, A*A FP64, int64 row_ptr",
+           "roofline": roofline_hbm(num_b, st2["t_num_ms"], "numeric pass (heavy rows: LDS bitmap windows)",
+                                    measured_traffic("rmat")),
+           "two_pass_bytes": sym_b + scan_b + num_b,
+           "two_pass_frac": round((sym_b + scan_b + num_b) / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
            "rows": A.nrows, "nnz_A": A.nnz, "nnz_C": nnz, "nprod": st["nprod"],
            "ms_per_step": round(ms, 3), "GFLOPs": round(2.0 * st["nprod"] / ms / 1e6, 1),
            "phases_ms": {"binning": round(st2["t_bin_ms"], 3), "symbolic": round(st2["t_sym_ms"], 3),
@@ -287,6 +335,115 @@ def rmat_leg(scale: int = 20, steps: int = 3, warmup: int = 1):
            "heavy_rows": {"symbolic": st["sym_bin"][8], "numeric": st["num_bin"][8]},
            "max_intprod": st["max_intprod"], "max_row_nnz": st["max_row_nnz"]}
     del C, rpt
+    torch.cuda.empty_cache()
+    return out
+
+
+def c1_leg(steps: int = 50, warmup: int = 5):
+    """The 5-point Laplacian on a 256^2 grid, C = A*A in FP64: the product
+    on the GPU (symbolic + scan + numeric per step) and the reference's host
+    path on this machine's cores (the oracle's binary COO reader of A and its
+    row-wise Gustavson product), results compared."""
+    import statistics
+    import tempfile
+    import torch
+    import nsp_amd
+    from oracle import oracle
+    A, B, desc = build_c1()
+    dA = nsp_amd.DeviceCSR.from_host(A)
+    h = nsp_amd.Handle()
+    h.set_param("timing", 1)
+    rpt, nnz = h.symbolic(dA, dA)
+    C = h.numeric(dA, dA, rpt, nnz)
+    nprod = h.stats()["nprod"]
+
+    def step():
+        r, z = h.symbolic(dA, dA)
+        h.numeric(dA, dA, r, z, out=nsp_amd.DeviceCSR(r, C.col, C.val, C.nrows, C.ncols))
+
+    ms = _timed(step, steps, warmup)
+    t_num = []
+    for _ in range(10):
+        step()
+        t_num.append(h.stats()["t_num_ms"])
+    t_num = float(np.median(t_num))
+    ref = oracle.spgemm(A, A)
+    same = (np.array_equal(C.row_ptr.cpu().numpy().astype(np.int64), ref[0])
+            and np.array_equal(C.col.cpu().numpy(), ref[1]) and np.array_equal(C.val.cpu().numpy(), ref[2]))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    oracle.set_threads(threads)
+    cpu = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        oracle.spgemm(A, A, with_abs=False)
+        cpu.append(time.perf_counter() - t0)
+    cpu_s = statistics.median(cpu[1:])
+    rd = []
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "a.bin")
+        oracle.write_binary_coo(path, A)
+        for _ in range(6):
+            t0 = time.perf_counter()
+            oracle.read_binary_coo(path, 1, 0)
+            rd.append(time.perf_counter() - t0)
+    sym_b, scan_b, num_b = algorithmic_bytes(A, A, nnz, s=8)
+    return {"workload": desc, "nprod": nprod, "nnz_C": nnz, "ms_per_step": round(ms, 4),
+            "GFLOPs": round(2.0 * nprod / ms / 1e6, 1),
+            "roofline": roofline_hbm(num_b, t_num, "numeric pass (column-bitmap kernels: B has 65,536 columns, "
+                                                   "so the hash kernels)"),
+            "two_pass_frac": round((sym_b + scan_b + num_b) / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "equal_to_oracle_bits": bool(same),
+            "cpu_baseline": {"value": round(2.0 * nprod / cpu_s / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+                             "kind": "port", "seconds": round(cpu_s, 5),
+                             "reader_seconds": round(statistics.median(rd[1:]), 5),
+                             "sample": "the whole product: row-wise Gustavson with a dense accumulator, OpenMP, "
+                                       "median of 5 runs; reader = binary COO read of A (one rank), median of 5"},
+            "speedup_vs_cpu": round(cpu_s * 1e3 / ms, 1)}
+
+
+def share_leg(m: int = 128, rank: int = 1, nprocs: int = 4, steps: int = 5, warmup: int = 1):
+    """One rank's whole local share of the 27-point stencil on m^3 (rank 1
+    of 4: halo planes on both sides), as nsparseMGPU_version1 runs it on one
+    device: A's row block with rank-frame columns, halo discovery and A's
+    column compaction, the completedP gather from the full P, P's
+    compaction, then the product.  Timed end to end (inputs in HBM) beside
+    the product alone (the reference's loc_time)."""
+    import torch
+    import nsp_amd
+    from nsp_amd import csr, gen
+    n = m ** 3
+    r0, r1 = csr.rank_rows(n, nprocs, rank)
+    P = gen.stencil_27pt(m, seed=0)
+    A_rank = P.row_slice(r0, r1, col_shift=r0)
+    dP = nsp_amd.DeviceCSR.from_host(P)
+    dAr = nsp_amd.DeviceCSR.from_host(A_rank)
+    del P
+    h = nsp_amd.Handle()
+
+    def run():
+        Al = csr.CSR.from_device(dAr, full_n=n, row_shift=r0)
+        Al.m = n
+        return csr.nsparse_mgpu_version1_local(Al, dP, h=h)
+
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    tot, loc = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        C, A_, P_, loc_time = run()
+        torch.cuda.synchronize()
+        tot.append(time.perf_counter() - t0)
+        loc.append(loc_time)
+    nprod = h.stats()["nprod"]
+    tot_ms, loc_ms = float(np.median(tot)) * 1e3, float(np.median(loc)) * 1e3
+    out = {"workload": f"27-point stencil {m}^3, rank {rank} of {nprocs}: compaction, completedP gather, product "
+                       f"(FP64)", "rows_A": A_.n, "shrinked_m_A": A_.m, "completedP_rows": P_.n,
+           "shrinked_m_P": P_.m, "nnz_C": C.nnz, "nprod": nprod,
+           "end_to_end_ms": round(tot_ms, 3), "product_loc_time_ms": round(loc_ms, 3),
+           "compaction_and_gather_ms": round(tot_ms - loc_ms, 3),
+           "end_to_end_GFLOPs": round(2.0 * nprod / tot_ms / 1e6, 1)}
+    del C, A_, P_, dP, dAr
     torch.cuda.empty_cache()
     return out
 
@@ -394,10 +551,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="fwd_bwd,batched,rmat,coo,galerkin",
-                    help="extra measurements after the headline: fwd_bwd, batched, rmat, coo, galerkin "
-                         "(comma list, or none)")
+    ap.add_argument("--legs", default="c1,share,fwd_bwd,batched,rmat,coo,galerkin",
+                    help="extra measurements after the headline: c1, share, fwd_bwd, batched, rmat, coo, "
+                         "galerkin (comma list, or none)")
     ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-param", action="append", default=[],
+                    help="launch parameter name=value of the batched leg (repeatable); given, the autotuner "
+                         "does not run (profiling runs)")
+    ap.add_argument("--batch-no-loop", action="store_true",
+                    help="skip the batched leg's per-matrix loop (profiling runs)")
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's pick")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy three times (the reference kernel for "
@@ -477,7 +639,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "rows_A": A.nrows, "cols_A": A.ncols, "rows_B": B.nrows,
                    "cols_B": B.ncols, "nnz_A": A.nnz, "nnz_B": B.nnz, "nnz_C": nnz,
-                   "nprod": nprod, "index": "int32", "parallelism": "single GPU"},
+                   "nprod": nprod, "index": "int32", "accumulate": "f64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "numeric hash-accumulate + in-row sort",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -494,10 +656,16 @@ def main():
     }
     del C, dA, dB, h
     legs = [x for x in args.legs.split(",") if x and x != "none"]
+    if "c1" in legs:
+        out["c1"] = c1_leg()
+    if "share" in legs:
+        out["share"] = share_leg()
+        torch.cuda.empty_cache()
     if "fwd_bwd" in legs:
         out["fwd_bwd"] = {dt: fwd_bwd_leg(dt) for dt in ("f64", "f32")}
     if "batched" in legs:
-        out["batched"] = batched_leg(args.batch_count)
+        bp = {kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.batch_param}
+        out["batched"] = batched_leg(args.batch_count, params=bp or None, loop=not args.batch_no_loop)
         torch.cuda.empty_cache()
     if "rmat" in legs:
         out["rmat"] = rmat_leg()

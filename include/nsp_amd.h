@@ -111,11 +111,15 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *   "heavy_blocks" blocks in flight for rows past 16384 products / 4096
  *                  entries (0 = one per CU)
  *   "reuse_extents" 1: nsp_spgemm_backward reuses the per-entry B row
- *                  extents, product offsets and C row-length check of the
+ *                  extents, product offsets and longest C row of the
  *                  previous call on this handle when A's column, A's and B's
  *                  row-pointer and C's buffers are the same (the caller
  *                  promises the patterns did not change); such calls do not
  *                  synchronise the host
+ *   "bwd_contrib"  how nsp_spgemm_backward forms dB when A^T's permutation
+ *                  is given: 1 from per-product contributions written by the
+ *                  dA pass (default), 0 by binary search in C's rows, 2
+ *                  the contributions only when C has rows past 32 entries
  *   "timing"       1 records HIP events around each phase (see nsp_stats_t)
  *   "coo_chunk_records"  records per streamed chunk of nsp_coo_read
  *                  (1..2^26, default 2^21: two 48 MiB pinned buffers) */

@@ -289,14 +289,23 @@ __global__ __launch_bounds__(256) void k_grad_a(int64_t MA, const int32_t *__res
     }
     wave_lds_sync();
     const int64_t a0 = (int64_t)arpt[i], a1 = (int64_t)arpt[i + 1];
-    for (int64_t p = a0 + glane; p < a1; p += GS) {
+    // product offsets of the row's entries: the row's first one read once,
+    // the rest by a group scan of the B row lengths (no per-entry gather)
+    int64_t run = w ? poff[a0] : 0;
+    for (int64_t c0 = a0; c0 < a1; c0 += GS) {
+      const int64_t p = c0 + glane;
+      const bool act = p < a1;
+      const int len = act ? ab_len[p] : 0;
+      const int incl = group_incl_scan<GS>(len, glane);
+      const int64_t pofs = run + incl - len;
+      run += group_last<GS>(incl);
+      if (!act) continue;
       const int64_t b0 = ab_b0[p];
-      const int len = ab_len[p];
       V s = (V)0;
       if (w) {
         // also the product's share of dB: w[product] = A_ik dC_ij
         const V a = aval[p];
-        V *wp = w + poff[p];
+        V *wp = w + pofs;
         for (int j = 0; j < len; j++) {
           const V d = hash_lookup_rt<V>(K, W, bcol[b0 + j], log);
           s += bval[b0 + j] * d;
@@ -419,15 +428,24 @@ __global__ __launch_bounds__(256) void k_grad_b(int64_t MB, const RowT *__restri
   }
 }
 
+// First product of the A entry behind each A^T entry, wpos[t] =
+// poff[perm[t]]: computed once per pattern, so that the dB pass reads one
+// coalesced array instead of gathering through the permutation.
+template <typename RowT>
+__global__ void k_wpos(int64_t n, const RowT *__restrict__ at_perm, const int64_t *__restrict__ poff,
+                       int64_t *__restrict__ wpos) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    wpos[t] = poff[(int64_t)at_perm[t]];
+}
+
 // dB from the products' contributions w (written by k_grad_a, numbered in
 // the forward's product order: A entry p owns products poff[p] ..
-// poff[p] + nnz(B_k) - 1): dB_kq = sum over A^T row k of w[poff[perm[t]] + q].
+// poff[p] + nnz(B_k) - 1): dB_kq = sum over A^T row k of w[wpos[t] + q].
 // Same chunking, lane split and fixed shuffle tree as k_grad_b.
 template <typename V, typename RowT, int SG>
 __global__ __launch_bounds__(256) void k_grad_b_sum(int64_t MB, const RowT *__restrict__ brpt,
-                                                    const RowT *__restrict__ atrpt, const RowT *__restrict__ at_perm,
-                                                    const int64_t *__restrict__ poff, const V *__restrict__ w,
-                                                    V *__restrict__ db) {
+                                                    const RowT *__restrict__ atrpt, const int64_t *__restrict__ wpos,
+                                                    const V *__restrict__ w, V *__restrict__ db) {
   constexpr int NSG = 256 / SG;
   __shared__ int64_t s_rpt[kGradBRows + 1];
   const int sg = threadIdx.x / SG, sl = threadIdx.x % SG;
@@ -448,7 +466,7 @@ __global__ __launch_bounds__(256) void k_grad_b_sum(int64_t MB, const RowT *__re
         }
         const int64_t k = k0 + lo, q = p - s_rpt[lo];
         const int64_t t0 = (int64_t)atrpt[k], t1 = (int64_t)atrpt[k + 1];
-        for (int64_t t = t0 + sl; t < t1; t += SG) s += w[poff[(int64_t)at_perm[t]] + q];
+        for (int64_t t = t0 + sl; t < t1; t += SG) s += w[wpos[t] + q];
       }
       s = group_sum<SG>(s);
       if (active && sl == 0) db[p] = s;
@@ -461,6 +479,19 @@ __global__ void k_zero_words(uint32_t *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
+// Longest row of a CSR (its row pointer) into *out (zeroed before).
+template <typename RowT>
+__global__ __launch_bounds__(256) void k_row_maxlen(int64_t rows, const RowT *__restrict__ rpt,
+                                                    unsigned long long *out) {
+  unsigned long long m = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long l = (unsigned long long)((int64_t)rpt[i + 1] - (int64_t)rpt[i]);
+    m = l > m ? l : m;
+  }
+  m = group_max<64>(m);
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
 template <typename V, typename RowT>
 static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const nsp_csr_t *AT,
                          const RowT *at_perm, const nsp_csr_t *C, const V *dc, V *da, V *db) {
@@ -471,12 +502,13 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
   const int64_t MA = A->rows, nnzA = A->nnz, nprod = h->ext_nprod;
   // dB from per-product contributions when A^T's permutation is given and
   // the contributions fit the budget, else by search in C's rows
-  const bool contrib = at_perm && nprod > 0 && (size_t)nprod * sizeof(V) <= kBackwardContribMax;
+  const bool contrib_ok = at_perm && nprod > 0 && (size_t)nprod * sizeof(V) <= kBackwardContribMax;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = nnzA / kScanTile + 2;
   const size_t o_rows = 256, o_status = o_rows + 3 * al((size_t)std::max<int64_t>(MA, 1) * 4),
-               o_poff = o_status + al((size_t)ntiles * 8), o_w = o_poff + al((size_t)(nnzA + 1) * 8),
-               need = contrib ? o_w + al((size_t)nprod * sizeof(V)) : o_poff;
+               o_poff = o_status + al((size_t)ntiles * 8), o_wpos = o_poff + al((size_t)(nnzA + 1) * 8),
+               o_w = o_wpos + al((size_t)std::max<int64_t>(nnzA, 1) * 8),
+               need = contrib_ok ? o_w + al((size_t)nprod * sizeof(V)) : o_poff;
   if (need > h->bwd.bytes) {
     h->poff_valid = false;
     if (h->bwd.base) NSP_HIP(h, hipFree(h->bwd.base));
@@ -490,15 +522,43 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
   }
   char *base = (char *)h->bwd.base;
   // [1] rows for the second tier, [2] scan ticket, [3] rows for the third
-  // tier, [4] rows for the search tier
+  // tier, [4] rows for the search tier, [6..7] longest C row
   uint32_t *flags = (uint32_t *)base;
   const size_t list_bytes = al((size_t)std::max<int64_t>(MA, 1) * 4);
   int32_t *mid_rows = (int32_t *)(base + o_rows);
   int32_t *long_rows = (int32_t *)(base + o_rows + list_bytes);
   int32_t *search_rows = (int32_t *)(base + o_rows + 2 * list_bytes);
+  // The longest C row decides which tiers run.  It is found once per C
+  // pattern (one host synchronisation); with reuse_extents and the same C
+  // buffers the cached value is used and the call does not synchronise.
+  const bool c_cached = h->reuse_extents && h->bwd_c_key[0] == C->row_ptr && h->bwd_c_key[1] == C->col &&
+                        h->bwd_c_rows == C->rows && h->bwd_c_nnz == C->nnz;
+  if (!c_cached) {
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 8);
+    if (C->rows > 0)
+      hipLaunchKernelGGL(k_row_maxlen<RowT>, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((C->rows + 255) / 256,
+                                                                                           (int64_t)h->num_cu * 4))),
+                         dim3(256), 0, st, C->rows, (const RowT *)C->row_ptr, (unsigned long long *)(flags + 6));
+    unsigned long long cm = 0;
+    NSP_HIP(h, hipMemcpyAsync(&cm, flags + 6, 8, hipMemcpyDeviceToHost, st));
+    NSP_HIP(h, hipStreamSynchronize(st));
+    h->bwd_cmax = (int64_t)cm;
+    h->bwd_c_key[0] = C->row_ptr;
+    h->bwd_c_key[1] = C->col;
+    h->bwd_c_rows = C->rows;
+    h->bwd_c_nnz = C->nnz;
+  }
+  const int64_t cmax = h->bwd_cmax;
+  // dB from the products' contributions (written by dA's pass; "bwd_contrib"
+  // 1, the default) or by search in C's rows (0), or contributions only when
+  // C has rows past 32 entries (2).  On the 64^3 stencil times the
+  // aggregation prolongator (C rows of at most 18 entries) the contributions
+  // measured faster: fwd+bwd 0.282 ms against 0.313 ms by search.
+  const bool contrib = contrib_ok && (h->bwd_contrib == 1 || (h->bwd_contrib == 2 && cmax > 32));
   int64_t *poff = contrib ? (int64_t *)(base + o_poff) : nullptr;
   V *w = contrib ? (V *)(base + o_w) : nullptr;
-  hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 5);
+  // the row lists of tiers 2-4 and the scan ticket start at zero
+  if (cmax > 32 || (contrib && !h->poff_valid)) hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, st, flags, 5);
   if (contrib && !h->poff_valid) {
     // product offsets of the A entries: exclusive scan of the B row lengths
     uint64_t *status = (uint64_t *)(base + o_status);
@@ -507,6 +567,14 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
     hipLaunchKernelGGL((k_exscan<int32_t, int64_t>), dim3((unsigned)(nnzA / kScanTile + 1)), dim3(kScanBS), 0, st,
                        nnzA, h->ab_len, poff, status, flags + 2);
     h->poff_valid = true;
+    h->wpos_key = nullptr;
+  }
+  int64_t *wpos = contrib ? (int64_t *)(base + o_wpos) : nullptr;
+  if (contrib && h->wpos_key != (const void *)at_perm && nnzA > 0) {
+    hipLaunchKernelGGL(k_wpos<RowT>, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((nnzA + 255) / 256,
+                                                                                          (int64_t)h->num_cu * 16))),
+                       dim3(256), 0, st, nnzA, at_perm, (const int64_t *)poff, wpos);
+    h->wpos_key = at_perm;
   }
   if (MA > 0) {
     // tier 1: C rows up to 32 entries (32 lanes per row, 64-slot tables);
@@ -519,18 +587,21 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
                        (const unsigned int *)nullptr, (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
                        (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
                        (const int64_t *)poff, w, mid_rows, flags + 1);
-    hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128>), dim3(h->num_cu * 2), dim3(256), 0, st, MA,
-                       (const int32_t *)mid_rows, (const unsigned int *)(flags + 1), (const RowT *)A->row_ptr,
-                       h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
-                       (const V *)A->val, (const int64_t *)poff, w, long_rows, flags + 3);
-    hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024>), dim3(h->num_cu), dim3(256), 0, st, MA,
-                       (const int32_t *)long_rows, (const unsigned int *)(flags + 3), (const RowT *)A->row_ptr,
-                       h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
-                       (const V *)A->val, (const int64_t *)poff, w, search_rows, flags + 4);
-    hipLaunchKernelGGL((k_grad_a_search<V, RowT>), dim3(h->num_cu), dim3(256), 0, st, (const int32_t *)search_rows,
-                       (const unsigned int *)(flags + 4), (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
-                       (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
-                       (const int64_t *)poff, w);
+    if (cmax > 32)
+      hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 128>), dim3(h->num_cu * 2), dim3(256), 0, st, MA,
+                         (const int32_t *)mid_rows, (const unsigned int *)(flags + 1), (const RowT *)A->row_ptr,
+                         h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
+                         (const V *)A->val, (const int64_t *)poff, w, long_rows, flags + 3);
+    if (cmax > 128)
+      hipLaunchKernelGGL((k_grad_a<V, RowT, 64, 1024>), dim3(h->num_cu), dim3(256), 0, st, MA,
+                         (const int32_t *)long_rows, (const unsigned int *)(flags + 3), (const RowT *)A->row_ptr,
+                         h->ab_b0, h->ab_len, B->col, (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da,
+                         (const V *)A->val, (const int64_t *)poff, w, search_rows, flags + 4);
+    if (cmax > 1024)
+      hipLaunchKernelGGL((k_grad_a_search<V, RowT>), dim3(h->num_cu), dim3(256), 0, st, (const int32_t *)search_rows,
+                         (const unsigned int *)(flags + 4), (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
+                         (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
+                         (const int64_t *)poff, w);
   }
   if (B->rows > 0 && B->nnz > 0) {
     // lanes per B entry from the mean A^T row length
@@ -540,8 +611,8 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
 #define GRAD_B(SG)                                                                                            \
   if (contrib)                                                                                                \
     hipLaunchKernelGGL((k_grad_b_sum<V, RowT, SG>), dim3(g), dim3(256), 0, st, B->rows,                       \
-                       (const RowT *)B->row_ptr, (const RowT *)AT->row_ptr, at_perm, (const int64_t *)poff,   \
-                       (const V *)w, db);                                                                     \
+                       (const RowT *)B->row_ptr, (const RowT *)AT->row_ptr, (const int64_t *)wpos, (const V *)w, \
+                       db);                                                                                   \
   else                                                                                                        \
     hipLaunchKernelGGL((k_grad_b<V, RowT, SG>), dim3(g), dim3(256), 0, st, B->rows, (const RowT *)B->row_ptr, \
                        B->col, (const RowT *)AT->row_ptr, AT->col, (const V *)AT->val, (const V *)A->val,     \

@@ -179,6 +179,9 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
     return rpw(&h->num_small_rows_per_wave, name, true);
   } else if (!std::strcmp(name, "num_rows_per_wave")) {
     return rpw(&h->num_rows_per_wave, name, false);
+  } else if (!std::strcmp(name, "bwd_contrib")) {
+    if (value < 0 || value > 2) return nsp::set_error(h, NSP_ERR_INVALID, "bwd_contrib must be 0, 1 or 2");
+    h->bwd_contrib = (int)value;
   } else if (!std::strcmp(name, "fp32_accumulate")) {
     if (value != 0 && value != 1) return nsp::set_error(h, NSP_ERR_INVALID, "fp32_accumulate must be 0 or 1");
     h->fp32_accumulate = (int)value;

@@ -94,6 +94,11 @@ struct Handle {
   size_t pin_bytes = 0;
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   bool poff_valid = false;  // bwd holds the product offsets of the cached extents
+  const void *wpos_key = nullptr;  // A^T permutation whose product positions bwd holds
+  // longest row of the C pattern the backward last ran on (its buffers and sizes)
+  const void *bwd_c_key[2] = {nullptr, nullptr};
+  int64_t bwd_c_rows = -1, bwd_c_nnz = -1, bwd_cmax = 0;
+  int bwd_contrib = 1;  // dB: 1 contributions, 0 search in C's rows, 2 by C's longest row
   Info *d_info = nullptr;
   Info *h_info = nullptr;  // pinned
 

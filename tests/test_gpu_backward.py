@@ -182,15 +182,19 @@ def test_backward_general_patterns(nsp, orc, case, dtype, use_perm):
     dc = rng.standard_normal(C.nnz)
     if dtype == "f32":
         dc = dc.astype(np.float32).astype(np.float64)
-    da, db = h.backward(dA_, dB_, AT, C, torch.from_numpy(dc).to(device="cuda", dtype=C.val.dtype),
-                        at_perm=perm)
     ccol = C.col.cpu().numpy()
     ref_da, ref_db = orc.backward(A, B, crpt, ccol, dc)
     abs_da, abs_db = orc.backward(abs_csr(A), abs_csr(B), crpt, ccol, np.abs(dc))
-    err_a = np.max(np.abs(da.cpu().numpy() - ref_da) / np.where(abs_da > 0, abs_da, 1))
-    err_b = np.max(np.abs(db.cpu().numpy() - ref_db) / np.where(abs_db > 0, abs_db, 1))
-    print(f"  componentwise error dA {err_a:.2e} dB {err_b:.2e}", flush=True)
-    assert err_a <= TOL[dtype] and err_b <= TOL[dtype]
+    # dB by per-product contributions (1), by search in C's rows (0), and
+    # the default choice by C's longest row (2, last: kept for below)
+    for mode in ((1, 0, 2) if use_perm else (2,)):
+        h.set_param("bwd_contrib", mode)
+        da, db = h.backward(dA_, dB_, AT, C, torch.from_numpy(dc).to(device="cuda", dtype=C.val.dtype),
+                            at_perm=perm)
+        err_a = np.max(np.abs(da.cpu().numpy() - ref_da) / np.where(abs_da > 0, abs_da, 1))
+        err_b = np.max(np.abs(db.cpu().numpy() - ref_db) / np.where(abs_db > 0, abs_db, 1))
+        print(f"  bwd_contrib {mode}: componentwise error dA {err_a:.2e} dB {err_b:.2e}", flush=True)
+        assert err_a <= TOL[dtype] and err_b <= TOL[dtype]
     # deterministic: a second call gives the same bits, also when it reuses
     # the first call's B row extents
     h.set_param("reuse_extents", 1)
