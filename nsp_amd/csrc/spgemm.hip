@@ -762,8 +762,12 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
   uint2 *Wb = wb[slot];
   AccN *Vs = vals[slot];
-  // adds a window's products into the value slots of their positions
-  auto accumulate = [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R]) {
+  // adds a window's products into the value slots of their positions; with
+  // `first` (the product that set its column's bit), that product also
+  // writes the column into C at its position, so each column is written
+  // exactly once and no pass over the bitmap is needed
+  auto accumulate = [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], const bool *first,
+                        int32_t *crow) {
     bool live[R];
 #pragma unroll
     for (int u = 0; u < R; u++) live[u] = __ballot(ok[u]) != 0;
@@ -775,6 +779,7 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
       const uint32_t rank = e.y + (uint32_t)__popc(e.x & ((1u << (k & 31)) - 1u));
       const uint32_t idx = ok[u] ? rank : (uint32_t)(NS + glane);
       __hip_atomic_fetch_add(&Vs[idx], (AccN)prod[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (first && first[u]) crow[rank] = (int32_t)k;
     }
   };
   const XcdTiles xt((count + RPB - 1) / RPB);
@@ -791,10 +796,11 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
       wave_lds_sync();
       const int64_t a0 = arpt[r], a1 = arpt[r + 1];
       // walk 1: the row's columns into the bitmap; a row that is one window
-      // long keeps its products in registers for the accumulation
+      // long keeps its products in registers for the accumulation, with a
+      // flag on the product that set its column's bit first
       int32_t sk[R];
       V sp[R];
-      bool so[R];
+      bool so[R], sf[R];
       int windows = 0;
       row_products<GS, R, true, V, true, O32>(
           glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
@@ -803,10 +809,12 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
             for (int u = 0; u < R; u++) {
               // lanes past the end OR nothing into a word of their own
               const uint32_t k = (uint32_t)key[u] & (kNarrowCols - 1);
-              atomicOr(&Wb[ok[u] ? (k >> 5) : (uint32_t)glane].x, ok[u] ? (1u << (k & 31)) : 0u);
+              const uint32_t bit = ok[u] ? (1u << (k & 31)) : 0u;
+              const uint32_t old = atomicOr(&Wb[ok[u] ? (k >> 5) : (uint32_t)glane].x, bit);
               sk[u] = key[u];
               sp[u] = prod[u];
               so[u] = ok[u];
+              sf[u] = ok[u] && !(old & bit);
             }
             windows++;
           });
@@ -828,31 +836,32 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
         }
       }
       wave_lds_sync();
+      const int64_t c0 = crpt[r];
+      const int nz = (int)(crpt[r + 1] - c0);
       if (windows == 1) {
-        accumulate(sk, sp, so);
+        accumulate(sk, sp, so, sf, ccol + c0);
       } else {
         // walk 2: each product into the slot of its column's position
         row_products<GS, R, true, V, true, O32>(
             glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
             [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
-              accumulate(key, prod, ok);
+              accumulate(key, prod, ok, nullptr, nullptr);
             });
-      }
-      wave_lds_sync();
-      // the row in column order: columns from the bits, values in order
-      const int64_t c0 = crpt[r];
-      const int nz = (int)(crpt[r + 1] - c0);
+        // the columns from the bits (positions from the running counts)
 #pragma unroll
-      for (int w = 0; w < WPL; w++) {
-        const uint2 e = Wb[glane * WPL + w];
-        uint32_t x = e.x, rk = e.y;
-        while (x) {
-          const int b = __ffs(x) - 1;
-          x &= x - 1u;
-          if ((int)rk < nz) ccol[c0 + rk] = (glane * WPL + w) * 32 + b;
-          rk++;
+        for (int w = 0; w < WPL; w++) {
+          const uint2 e = Wb[glane * WPL + w];
+          uint32_t x = e.x, rk = e.y;
+          while (x) {
+            const int b = __ffs(x) - 1;
+            x &= x - 1u;
+            if ((int)rk < nz) ccol[c0 + rk] = (glane * WPL + w) * 32 + b;
+            rk++;
+          }
         }
       }
+      wave_lds_sync();
+      // the values in column order
       for (int j = glane; j < nz; j += GS) cval[c0 + j] = (V)Vs[j];
       wave_lds_sync();
     }
