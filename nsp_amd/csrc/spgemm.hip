@@ -949,7 +949,8 @@ constexpr int kHeavyBS = 1024;
 constexpr int kHeavyNW = kHeavyBS / 64;
 constexpr int64_t kSymHeavyWin = 1 << 20;   // columns per symbolic window (128 KB of bits)
 constexpr int64_t kNumHeavyWin = 1 << 20;   // columns per numeric window (128 KB bits + 8 KB group ranks)
-constexpr int kRankGroup = 16;              // bitmap words per stored rank
+constexpr int kRankGroup = 8;               // bitmap words per stored rank (two 16-byte reads)
+constexpr int kRankSuper = 64;              // rank groups per 32-bit base; group ranks are 16-bit offsets
 
 // Smallest and largest column of a row's products.
 template <int BS>
@@ -1053,7 +1054,13 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
   constexpr int WPT = NWORD / kHeavyBS;  // consecutive words per thread in the rank scan
   static_assert(WPT % kRankGroup == 0, "a thread's words are whole rank groups");
   __shared__ __align__(16) uint32_t bm[NWORD];
-  __shared__ uint32_t gbase[NWORD / kRankGroup];  // row position of the first column of each word group
+  // row position of the first column of each word group: a 32-bit base per
+  // super-group of kRankSuper groups plus a 16-bit offset per group (a
+  // super-group covers 2^14 columns, so the offset fits)
+  static_assert(kRankGroup * kRankSuper * 32 <= 65536, "group offsets fit 16 bits");
+  static_assert((kRankGroup * kRankSuper) % WPT == 0, "a super-group is whole threads' words");
+  __shared__ uint16_t gbase[NWORD / kRankGroup];
+  __shared__ uint32_t sbase[NWORD / (kRankGroup * kRankSuper)];
   __shared__ EntryScratch<64, V> ent[kHeavyNW];
   __shared__ int64_t s_red[2 * kHeavyNW];
   __shared__ int s_scan[kHeavyNW + 1];
@@ -1093,11 +1100,15 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
       for (int q = 0; q < WPT; q++) mine += __popc(bm[q0 + q]);
       int wtotal;
       int run = block_excl_scan<kHeavyBS, int>(mine, s_scan, &wtotal);
+      // the super-group's base is the run of its first thread
+      constexpr int TPS = kRankGroup * kRankSuper / WPT;  // threads per super-group
+      const int sb = __shfl(run, lane & ~(TPS - 1));
+      if ((threadIdx.x & (TPS - 1)) == 0) sbase[threadIdx.x / TPS] = (uint32_t)run;
       // sorted columns of the window, values zeroed for the accumulation
 #pragma unroll 1
       for (int q = 0; q < WPT; q++) {
         uint32_t x = bm[q0 + q];
-        if (q % kRankGroup == 0) gbase[(q0 + q) / kRankGroup] = (uint32_t)run;
+        if (q % kRankGroup == 0) gbase[(q0 + q) / kRankGroup] = (uint16_t)(run - sb);
         while (x) {
           const int b = __ffs(x) - 1;
           x &= x - 1;
@@ -1118,11 +1129,11 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
               const int64_t c = (int64_t)key[u] - w0;
               if (ok[u] && c >= 0 && c < kNumHeavyWin) {
                 // rank = group base + set bits of the group's words before
-                // this column (four 16-byte LDS reads, masked popcounts)
+                // this column (two 16-byte LDS reads, masked popcounts)
                 const int q = (int)(c >> 5), b = (int)(c & 31);
                 const int g = q / kRankGroup, j = q % kRankGroup;
                 const uint4 *G = reinterpret_cast<const uint4 *>(&bm[g * kRankGroup]);
-                int cnt = (int)gbase[g];
+                int cnt = (int)sbase[g / kRankSuper] + (int)gbase[g];
 #pragma unroll
                 for (int v4 = 0; v4 < kRankGroup / 4; v4++) {
                   const uint4 x = G[v4];
