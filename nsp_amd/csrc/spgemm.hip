@@ -948,6 +948,7 @@ __global__ __launch_bounds__(BS) void k_num_block(int bin_lo, int bin_hi, int64_
 constexpr int kHeavyBS = 1024;
 constexpr int kHeavyNW = kHeavyBS / 64;
 constexpr int64_t kSymHeavyWin = 1 << 20;   // columns per symbolic window (128 KB of bits)
+constexpr int kHeavyStage = 256;         // C entries a wave stages in LDS per copy-out
 constexpr int64_t kNumHeavyWin = 1 << 20;   // columns per numeric window (128 KB bits + 8 KB group ranks)
 constexpr int kRankGroup = 8;               // bitmap words per stored rank (two 16-byte reads)
 constexpr int kRankSuper = 64;              // rank groups per 32-bit base; group ranks are 16-bit offsets
@@ -1051,17 +1052,18 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
                                                         int32_t *__restrict__ ccol,
                                                         V *__restrict__ cval) {
   constexpr int NWORD = (int)(kNumHeavyWin / 32);
-  constexpr int WPT = NWORD / kHeavyBS;  // consecutive words per thread in the rank scan
-  static_assert(WPT % kRankGroup == 0, "a thread's words are whole rank groups");
+  constexpr int WPW = NWORD / kHeavyNW;  // consecutive words per wave in the rank scan
+  static_assert(64 % kRankGroup == 0 && WPW % (kRankGroup * kRankSuper) == 0,
+                "rank groups and super-groups start on a wave's 64-word steps");
   __shared__ __align__(16) uint32_t bm[NWORD];
   // row position of the first column of each word group: a 32-bit base per
   // super-group of kRankSuper groups plus a 16-bit offset per group (a
   // super-group covers 2^14 columns, so the offset fits)
   static_assert(kRankGroup * kRankSuper * 32 <= 65536, "group offsets fit 16 bits");
-  static_assert((kRankGroup * kRankSuper) % WPT == 0, "a super-group is whole threads' words");
   __shared__ uint16_t gbase[NWORD / kRankGroup];
   __shared__ uint32_t sbase[NWORD / (kRankGroup * kRankSuper)];
   __shared__ EntryScratch<64, V> ent[kHeavyNW];
+  static_assert(sizeof(EntryScratch<64, V>) >= kHeavyStage * sizeof(int32_t), "the output stage fits a wave's scratch");
   __shared__ int64_t s_red[2 * kHeavyNW];
   __shared__ int s_scan[kHeavyNW + 1];
   int64_t count, off;
@@ -1093,33 +1095,60 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
             }
           });
       __syncthreads();
-      // ranks: exclusive scan of per-word popcounts (WPT consecutive words per thread)
-      const int q0 = threadIdx.x * WPT;
+      // ranks: wave w owns words [w*WPW, (w+1)*WPW), taken 64 at a time with
+      // lane l on word i*64 + l, so that the columns a wave writes in one
+      // step are consecutive entries of C (coalesced stores)
+      const int qw = w * WPW;
       int mine = 0;
 #pragma unroll
-      for (int q = 0; q < WPT; q++) mine += __popc(bm[q0 + q]);
+      for (int i = 0; i < WPW / 64; i++) mine += __popc(bm[qw + i * 64 + lane]);
       int wtotal;
-      int run = block_excl_scan<kHeavyBS, int>(mine, s_scan, &wtotal);
-      // the super-group's base is the run of its first thread
-      constexpr int TPS = kRankGroup * kRankSuper / WPT;  // threads per super-group
-      const int sb = __shfl(run, lane & ~(TPS - 1));
-      if ((threadIdx.x & (TPS - 1)) == 0) sbase[threadIdx.x / TPS] = (uint32_t)run;
-      // sorted columns of the window, values zeroed for the accumulation
+      int run = __shfl(block_excl_scan<kHeavyBS, int>(mine, s_scan, &wtotal), 0);  // entries before the wave
+      int sb = 0;  // row position of the current super-group's first column
+      // sorted columns of the window, values zeroed for the accumulation:
+      // the lanes place a step's columns in a per-wave LDS stage (the
+      // enumerator's scratch, idle between the walks), kHeavyStage at a time,
+      // and the wave copies the stage out with consecutive lanes on
+      // consecutive entries, whatever the density of the step's words
+      int32_t *stage = reinterpret_cast<int32_t *>(&ent[w]);
 #pragma unroll 1
-      for (int q = 0; q < WPT; q++) {
-        uint32_t x = bm[q0 + q];
-        if (q % kRankGroup == 0) gbase[(q0 + q) / kRankGroup] = (uint16_t)(run - sb);
-        while (x) {
-          const int b = __ffs(x) - 1;
-          x &= x - 1;
-          if (base + run < nzr) {  // the symbolic count bounds the row
-            ccol[c0 + base + run] = (int32_t)(w0 + (int64_t)(q0 + q) * 32 + b);
-            cval[c0 + base + run] = (V)0;
-          }
-          run++;
+      for (int i = 0; i < WPW / 64; i++) {
+        const int q = qw + i * 64 + lane;
+        uint32_t x = bm[q];
+        const int p = __popc(x);
+        const int incl = group_incl_scan<64>(p, lane);
+        const int E = group_last<64>(incl);  // entries of the step
+        int pos = incl - p;                  // lane's first entry, from the step's first
+        if (i % (kRankGroup * kRankSuper / 64) == 0) {  // a super-group starts at lane 0
+          sb = run;
+          if (lane == 0) sbase[q / (kRankGroup * kRankSuper)] = (uint32_t)run;
         }
+        if (lane % kRankGroup == 0) gbase[q / kRankGroup] = (uint16_t)(run + pos - sb);
+        const int32_t colq = (int32_t)(w0 + (int64_t)q * 32);
+        for (int r0 = 0; r0 < E; r0 += kHeavyStage) {
+          while (x && pos < r0 + kHeavyStage) {
+            const int b = __ffs(x) - 1;
+            x &= x - 1;
+            stage[pos - r0] = colq + b;
+            pos++;
+          }
+          wave_lds_sync();
+          const int n = E - r0 < kHeavyStage ? E - r0 : kHeavyStage;
+          for (int j = lane; j < n; j += 64) {
+            const int64_t k = base + run + r0 + j;
+            if (k < nzr) {  // the symbolic count bounds the row
+              ccol[c0 + k] = stage[j];
+              cval[c0 + k] = (V)0;
+            }
+          }
+          wave_lds_sync();
+        }
+        run += E;
       }
-      __threadfence();
+      // the zeroed values are in this XCD's L2, where the block's atomics
+      // below land: the barrier's workgroup-scope release (wait for the
+      // stores) orders them; an agent-scope fence would write back and
+      // invalidate the whole L2 on every row
       __syncthreads();
       row_products<64, R, true, V>(
           lane, a0, a1, 64 * w, 64 * kHeavyNW, ab_b0, ab_len, aval, bcol, bval, &ent[w],
