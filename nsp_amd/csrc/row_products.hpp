@@ -52,6 +52,33 @@ struct Window {
   bool ok[R];
 };
 
+// A lane's A entry of a row's first chunk of GS entries (its B row start and
+// length, and its value), loaded ahead of the row: the group kernels load
+// the next row's first chunk while the current row is hashed and sorted, so
+// a row starts on its B loads instead of on two dependent global round
+// trips (row extent, then entry extents).
+template <typename V>
+struct FirstChunk {
+  int64_t b0 = 0;
+  int len = 0;
+  V av = (V)0;
+};
+
+template <bool VAL, typename V>
+__device__ __forceinline__ FirstChunk<V> load_first_chunk(int glane, int64_t a0, int64_t a1,
+                                                         const int64_t *__restrict__ ab_b0,
+                                                         const int32_t *__restrict__ ab_len,
+                                                         const V *__restrict__ aval) {
+  FirstChunk<V> f;
+  const int64_t ja = a0 + glane;
+  if (ja < a1) {
+    f.b0 = ab_b0[ja];
+    f.len = ab_len[ja];
+    if (VAL) f.av = aval[ja];
+  }
+  return f;
+}
+
 // f(key[R], product[R], ok[R], bval[R], entry[R]) is called once per window
 // with lane g's R products of the window; ok[u] is false past the end of the
 // entries, entry[u] is the product's compacted entry index in the chunk.
@@ -64,6 +91,8 @@ struct Window {
 // [c0, c0+GS), c0 = a0 + first, first + step, ... (step = GS for a group that
 // owns the row; a block splits a row's chunks over its waves with
 // first = wave*64, step = 64*waves).
+// With `pre` (first == 0), the row's first chunk comes from *pre instead of
+// being loaded.
 template <int GS, int R, bool VAL, typename V, bool LOADB = VAL, bool O32 = false, typename F>
 __device__ __forceinline__ void row_products(const int glane, const int64_t a0, const int64_t a1,
                                              const int64_t first, const int64_t step,
@@ -72,7 +101,8 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
                                              const V *__restrict__ aval,
                                              const int32_t *__restrict__ bcol,
                                              const V *__restrict__ bval,
-                                             EntryScratch<GS, V> *__restrict__ es, F &&f) {
+                                             EntryScratch<GS, V> *__restrict__ es, F &&f,
+                                             const FirstChunk<V> *pre = nullptr) {
   static_assert(kMaskWords % R == 0, "a window must not straddle the mask words");
   const uint64_t le = (glane == 63) ? ~0ull : ((2ull << glane) - 1);
   const uint64_t lt = (1ull << glane) - 1;
@@ -81,7 +111,11 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
     int64_t b0 = 0;
     int len = 0;
     V av = (V)0;
-    if (ja < a1) {
+    if (pre && c0 == a0) {
+      b0 = pre->b0;
+      len = pre->len;
+      av = pre->av;
+    } else if (ja < a1) {
       b0 = ab_b0[ja];
       len = ab_len[ja];
       if (VAL) av = aval[ja];
@@ -200,3 +234,4 @@ __device__ __forceinline__ void row_products(const int glane, const int64_t a0, 
 }
 
 }  // namespace nsp
+

@@ -430,23 +430,41 @@ void k_sym_group(int bin_lo, int bin_hi, int64_t direct_rows,
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
   int32_t *K = keys[slot];
   const XcdTiles xt((count + RPB - 1) / RPB);
-  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
-    const int64_t base = tile * RPB;
-    const int64_t t = base + slot;
-    if (t < count) {
-      const int32_t r = perm ? perm[off + t] : (int32_t)t;
-      clear_lds<GS, SHD>(K, glane, kEmpty);
-      wave_lds_sync();
-      int cnt = 0;
-      row_products<GS, R, false, float, false, O32>(
-          glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[slot],
-          [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
-            cnt += hash_insert_keys<LOG, R>(K, key, ok, glane);
-          });
-      cnt = group_sum<GS>(cnt);
-      if (glane == 0) row_nz[r] = cnt;
-      wave_lds_sync();
+  // the group's rows, one tile apart; the next row's A extent and first
+  // chunk of entries are loaded while the current row is hashed
+  auto row_at = [&](int64_t tile) -> int32_t {
+    const int64_t t = tile * RPB + slot;
+    return (tile < xt.end && t < count) ? (perm ? perm[off + t] : (int32_t)t) : -1;
+  };
+  int32_t rn = row_at(xt.first);
+  int64_t an0 = 0, an1 = 0;
+  FirstChunk<float> pre;
+  if (rn >= 0) {
+    an0 = arpt[rn];
+    an1 = arpt[rn + 1];
+    pre = load_first_chunk<false, float>(glane, an0, an1, ab_b0, ab_len, nullptr);
+  }
+  for (int64_t tile = xt.first; rn >= 0; tile += xt.step) {
+    const int32_t r = rn;
+    const int64_t a0 = an0, a1 = an1;
+    const FirstChunk<float> cur = pre;
+    rn = row_at(tile + xt.step);
+    if (rn >= 0) {
+      an0 = arpt[rn];
+      an1 = arpt[rn + 1];
     }
+    clear_lds<GS, SHD>(K, glane, kEmpty);
+    wave_lds_sync();
+    int cnt = 0;
+    auto ins = [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
+      cnt += hash_insert_keys<LOG, R>(K, key, ok, glane);
+    };
+    row_products<GS, R, false, float, false, O32>(glane, a0, a1, 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr,
+                                                  &ent[slot], ins, &cur);
+    if (rn >= 0) pre = load_first_chunk<false, float>(glane, an0, an1, ab_b0, ab_len, nullptr);
+    cnt = group_sum<GS>(cnt);
+    if (glane == 0) row_nz[r] = cnt;
+    wave_lds_sync();
   }
 }
 
@@ -601,27 +619,52 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
   Acc *W = vals[slot];
   uint32_t *S = stage[slot];
   const XcdTiles xt((count + RPB - 1) / RPB);
-  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
-    const int64_t base = tile * RPB;
-    const int64_t t = base + slot;
-    if (t < count) {
-      const int32_t r = perm ? perm[off + t] : (int32_t)t;
-      if (crpt[r + 1] > info->cap) {  // C buffers too small: skip the row, flag it
+  // the group's rows, one tile apart; the next row's A extent is loaded at
+  // the top of the current one and its first chunk of entries after the
+  // current row's products, so both arrive while this row is hashed, sorted
+  // and stored
+  auto row_at = [&](int64_t tile) -> int32_t {
+    const int64_t t = tile * RPB + slot;
+    return (tile < xt.end && t < count) ? (perm ? perm[off + t] : (int32_t)t) : -1;
+  };
+  int32_t rn = row_at(xt.first);
+  int64_t an0 = 0, an1 = 0;
+  FirstChunk<V> pre;
+  if (rn >= 0) {
+    an0 = arpt[rn];
+    an1 = arpt[rn + 1];
+    pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
+  }
+  for (int64_t tile = xt.first; rn >= 0; tile += xt.step) {
+    {
+      const int32_t r = rn;
+      const int64_t a0 = an0, a1 = an1;
+      const FirstChunk<V> cur = pre;
+      rn = row_at(tile + xt.step);
+      if (rn >= 0) {
+        an0 = arpt[rn];
+        an1 = arpt[rn + 1];
+      }
+      const int64_t c0 = crpt[r];
+      const int nz = (int)(crpt[r + 1] - c0);
+      if (c0 + nz > info->cap) {  // C buffers too small: skip the row, flag it
         if (glane == 0) atomicOr((unsigned int *)&const_cast<Info *>(info)->error, (unsigned int)NSP_ERR_OVERFLOW);
+        if (rn >= 0) pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
         continue;
       }
       clear_lds<GS, SHD>(K, glane, kEmpty);
       clear_lds<GS, SHD>(W, glane, 0);
       wave_lds_sync();
-      row_products<GS, R, true, V, true, O32>(glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, aval, bcol, bval,
-                                   &ent[slot],
-                                   [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
-                                     Acc pa[R];
+      auto acc = [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
+        Acc pa[R];
 #pragma unroll
-                                     for (int u = 0; u < R; u++) pa[u] = (Acc)prod[u];
-                                     hash_accumulate_batch<LOG, R, Acc>(K, W, key, pa, ok, glane);
-                                   });
+        for (int u = 0; u < R; u++) pa[u] = (Acc)prod[u];
+        hash_accumulate_batch<LOG, R, Acc>(K, W, key, pa, ok, glane);
+      };
+      row_products<GS, R, true, V, true, O32>(glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
+                                              acc, &cur);
       wave_lds_sync();
+      if (rn >= 0) pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
       // compact occupied slots (strided: slot q*GS + glane) into S; with
       // `pack` (C's columns below 2^(32-LOG)) the key carries its slot in
       // the low LOG bits, so the sorted entries read their values directly
@@ -635,8 +678,6 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
         if (k != kEmpty) S[pos++] = pack ? (((uint32_t)k << LOG) | (uint32_t)(q * GS + glane)) : (uint32_t)k;
       }
       wave_lds_sync();
-      const int64_t c0 = crpt[r];
-      const int nz = (int)(crpt[r + 1] - c0);
       uint32_t v[E];
 #pragma unroll
       for (int q = 0; q < E; q++) {

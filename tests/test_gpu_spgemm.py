@@ -111,6 +111,51 @@ def test_rows_per_wave_and_table_variants(nsp, orc, rpw, dtype):
         check_against_oracle(A, B, C, dtype, orc, ref=ref)
 
 
+def bounded_rows(nrows, ncols, maxlen, seed, empty=0.0):
+    """Rows of uniform length in [0, maxlen], a fraction `empty` of them
+    emptied, columns uniform."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, maxlen + 1, nrows)
+    lens[rng.random(nrows) < empty] = 0
+    rows = np.repeat(np.arange(nrows), lens)
+    cols = np.concatenate([rng.choice(ncols, int(k), replace=False) for k in lens])
+    vals = rng.uniform(-1, 1, rows.size)
+    return gen._from_rows_cols(nrows, ncols, rows, cols, vals)
+
+
+@pytest.mark.parametrize("maxlen", [16, 32, 64])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_short_b_rows_empty_last_row(nsp, orc, maxlen, dtype):
+    """B rows of up to 16, 32 or 64 entries, a tenth of them empty, the last
+    one empty too (its row pointer entry is nnz(B), so a product walk that
+    loaded at an empty row's start would read past B): the hash kernels of
+    every group size and both numeric table factors, rows in every
+    wave-level bin, and a 27-point stencil (B rows of 27)."""
+    print("case: short B rows", maxlen, dtype, flush=True)
+    A = to_dtype(mixed_rows(2000, 1800, 1, 120, seed=11), dtype)
+    B = to_dtype(bounded_rows(1800, 6000, maxlen, seed=12, empty=0.1), dtype)
+    rp = B.rpt.copy()
+    rp[-1] = rp[-2]
+    B = gen.HostCSR(rp, B.col[:rp[-1]], B.val[:rp[-1]], B.nrows, B.ncols)
+    assert (A.col == B.nrows - 1).any()
+    ref = orc.spgemm(A, B)
+    for rpw in (1, 2, 4):
+        for ntf in (2, 4):
+            h = nsp.Handle()
+            for name in ("sym_small_rows_per_wave", "sym_rows_per_wave", "num_small_rows_per_wave",
+                         "num_rows_per_wave"):
+                h.set_param(name, rpw)
+            h.set_param("narrow_mode", 0)
+            h.set_param("num_table_factor", ntf)
+            C, _ = run(nsp, A, B, dtype, handle=h)
+            st = h.stats()
+            assert sum(st["sym_bin"][:6]) > 0 and sum(st["num_bin"][:5]) > 0
+            check_against_oracle(A, B, C, dtype, orc, ref=ref)
+    S = to_dtype(gen.stencil_27pt(24, seed=13), dtype)
+    C, _ = run(nsp, S, S, dtype)
+    check_against_oracle(S, S, C, dtype, orc)
+
+
 def test_structural_zeros_and_empty_rows(nsp, orc):
     print("case: zeros and empty rows", flush=True)
     A = gen.random_csr(400, 300, 0.02, seed=9, integer_values=True, empty_rows=0.3)
