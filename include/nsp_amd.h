@@ -334,6 +334,52 @@ int nsp_coo_to_csr(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const void *r
 int nsp_coo_read(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const char *path, int header_type,
                  const nsp_coo_share_t *share, nsp_csr_t *out);
 
+/* ---- ASCII COO files (the reference's text format) --------------------- */
+/* File layout (src/READ_MATRIX_MPI.cpp:364-578, read when load_MatrixMPI's
+ * `ascii` is set, src/distribute.cu:1084-1086): lines starting with '%',
+ * then a header line "nr nc nt" (or "nr nt", nc = nr), then nt record lines
+ * "i j v" (1-based, sorted by row) that all have the length of the first
+ * one, so record q starts at body_offset + q*line_bytes.  The column field
+ * is read from col_offset and the value from val_offset, found in the first
+ * record line as the reference finds them (the first space after the row
+ * field, the first space after the column field).  Rows and columns are
+ * read as atol reads them; values with the reference's own decimal parser
+ * (fast_atof, src/READ_MATRIX_MPI.cpp:21-77: digits accumulated in FP64 as
+ * value*10 + d, fraction digits added as d/10^k, the exponent applied as
+ * factors 1e50, 1e8 and 10), so every value is bit-identical to the one
+ * the reference reads.  Same rank split and col_shift as the binary files. */
+typedef struct {
+  int64_t nr, nc, nt;            /* the header */
+  int64_t body_offset;           /* first byte of record 0 */
+  int64_t line_bytes;            /* bytes per record line, newline included */
+  int32_t col_offset, val_offset;
+  int64_t rec_begin, rec_end;    /* the rank's records [rec_begin, rec_end) */
+  int64_t first_row;             /* 0-based row of the rank's first record */
+  int64_t nrows;                 /* rows spanned, first to last present row (rowxproc) */
+  int64_t col_shift;             /* sum of nrows over the ranks before this one */
+} nsp_ascii_share_t;
+
+/* Host only: reads the header and the first record line, checks that the
+ * file holds nt lines of that length, and locates the rank's records by
+ * binary search on the row field (the reference's binsearch_iat_str,
+ * src/READ_MATRIX_MPI.cpp:83-104, and its split, 440-455).  The body
+ * starts after the '%' lines and the header line (the reference counts only
+ * the header line's length, so it reads a file with '%' lines from the
+ * wrong offset; files without them read the same).  NSP_ERR_INVALID for a
+ * missing header, a missing or empty first record line, or a file too short
+ * for nt lines. */
+int nsp_ascii_share(const char *path, int nprocs, int rank, nsp_ascii_share_t *out);
+
+/* The rank's block as a device CSR, exactly as nsp_coo_read builds it from
+ * the same records (row pointer, columns in the rank frame, values FP64 or
+ * rounded to FP32, unsorted rows sorted stably): the record lines are read
+ * in chunks of "coo_chunk_records" lines through pinned buffers and parsed
+ * on the device, one lane per line.  A record line whose last byte is not a
+ * newline (lines of unequal length, which the reference would read
+ * misaligned) returns NSP_ERR_INVALID, as do records out of row order. */
+int nsp_ascii_read(nsp_handle_t h, nsp_index_t it, nsp_value_t vt, const char *path,
+                   const nsp_ascii_share_t *share, nsp_csr_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -216,8 +216,12 @@ CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type 
 /* The same block as a device CSR built on the device (nsp_coo_read: records
  * stream from the file through pinned buffers and are converted there), as
  * the reference's load_MatrixMPI returns it (src/distribute.cu:1071-1125,
- * binary files, row partition). */
-CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type = 1);
+ * row partition).  With `ascii` the file is the reference's text format
+ * (READ_ASCII_MATRIX_MPI, src/READ_MATRIX_MPI.cpp:364-578, taken when
+ * load_MatrixMPI's `ascii` is set, distribute.cu:1084-1086), parsed on the
+ * device by nsp_ascii_read; header_type is then unused (the text header
+ * carries its own field count). */
+CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type = 1, bool ascii = false);
 /* Writes a host CSR as binary COO (header_type 1); global column =
  * col + col_shift.  The layout the reader takes: the reference's
  * write_binary_matrix template (main.cu:468-512) with otype = long. */

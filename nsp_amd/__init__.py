@@ -309,6 +309,35 @@ class Handle:
                                         C.byref(s), C.byref(o)))
         return s, out
 
+    def ascii_read(self, path: str, nprocs: int = 1, rank: int = 0, index: str = "i32", dtype: str = "f64",
+                   share=None):
+        """(share, DeviceCSR): rank `rank` of `nprocs` of an ASCII COO file
+        parsed and built on the device (nsp_ascii_read); share as
+        ascii_share returns it."""
+        torch = _torch()
+        s = share if share is not None else ascii_share(path, nprocs, rank)
+        nrec = s.rec_end - s.rec_begin
+        dev = torch.device("cuda", torch.cuda.current_device())
+        out = DeviceCSR(torch.empty(s.nrows + 1, dtype=torch.int64 if index == "i64" else torch.int32,
+                                    device=dev),
+                        torch.empty(nrec, dtype=torch.int32, device=dev),
+                        torch.empty(nrec, dtype=torch.float32 if dtype == "f32" else torch.float64, device=dev),
+                        int(s.nrows), int(s.nc))
+        o = out.c_struct()
+        self._check(self.L.nsp_ascii_read(self.h, out.index_type, out.value_type, path.encode(), C.byref(s),
+                                          C.byref(o)))
+        return s, out
+
+
+def ascii_share(path: str, nprocs: int = 1, rank: int = 0) -> "_lib.nsp_ascii_share_t":
+    """Header, line layout and rank range of an ASCII COO file
+    (nsp_ascii_share; host only, no GPU needed)."""
+    s = _lib.nsp_ascii_share_t()
+    st = _lib.load().nsp_ascii_share(path.encode(), int(nprocs), int(rank), C.byref(s))
+    if st != _lib.NSP_OK:
+        raise NspError(st, f"cannot read the header, record lines or rows of {path}")
+    return s
+
 
 def coo_share(path: str, nprocs: int = 1, rank: int = 0, header_type: int = 1) -> "_lib.nsp_coo_share_t":
     """Header and rank range of a binary COO file (nsp_coo_share; host

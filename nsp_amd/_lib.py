@@ -38,6 +38,13 @@ class nsp_coo_share_t(C.Structure):
                 ("nrows", C.c_int64), ("col_shift", C.c_int64)]
 
 
+class nsp_ascii_share_t(C.Structure):
+    _fields_ = [("nr", C.c_int64), ("nc", C.c_int64), ("nt", C.c_int64), ("body_offset", C.c_int64),
+                ("line_bytes", C.c_int64), ("col_offset", C.c_int32), ("val_offset", C.c_int32),
+                ("rec_begin", C.c_int64), ("rec_end", C.c_int64), ("first_row", C.c_int64),
+                ("nrows", C.c_int64), ("col_shift", C.c_int64)]
+
+
 class NspError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"{status_string(status)}: {msg}")
@@ -79,6 +86,8 @@ SIGNATURES = [
     ("nsp_coo_share", _I, [C.c_char_p, _I, _I, _I, C.POINTER(nsp_coo_share_t)]),
     ("nsp_coo_to_csr", _I, [_H, _I, _I, _VP, _I64, _I64, _I64, _CSRP]),
     ("nsp_coo_read", _I, [_H, _I, _I, C.c_char_p, _I, C.POINTER(nsp_coo_share_t), _CSRP]),
+    ("nsp_ascii_share", _I, [C.c_char_p, _I, _I, C.POINTER(nsp_ascii_share_t)]),
+    ("nsp_ascii_read", _I, [_H, _I, _I, C.c_char_p, C.POINTER(nsp_ascii_share_t), _CSRP]),
 ]
 
 

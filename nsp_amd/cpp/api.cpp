@@ -429,7 +429,22 @@ CSR *read_binary_matrix(const char *path, int nprocs, int rank, int header_type,
   return A;
 }
 
-CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type) {
+CSR *load_MatrixMPI(const char *path, int nprocs, int rank, int header_type, bool ascii) {
+  if (ascii) {
+    nsp_ascii_share_t s{};
+    const int rc = nsp_ascii_share(path, nprocs, rank, &s);
+    if (rc != NSP_OK) throw Error(rc, std::string("load_MatrixMPI: cannot read the ASCII file ") + path);
+    const int n = (int)s.nrows, nnz = (int)(s.rec_end - s.rec_begin);
+    CSR *A = CSRm::init((stype)n, (gstype)s.nc, (stype)nnz, true, true, false, (gstype)s.nr, (gstype)s.col_shift);
+    nsp_handle_t h = handle();
+    nsp_csr_t out{n, s.nc, nnz, A->row, A->col, A->val};
+    const int st = nsp_ascii_read(h, NSP_IDX32, NSP_F64, path, &s, &out);
+    if (st != NSP_OK) {
+      CSRm::free(A);
+      check(st, h, "load_MatrixMPI");
+    }
+    return A;
+  }
   const nsp_coo_share_t s = coo_share(path, nprocs, rank, header_type, "load_MatrixMPI");
   const int n = (int)s.nrows, nnz = (int)(s.rec_end - s.rec_begin);
   CSR *A = CSRm::init((stype)n, (gstype)s.nc, (stype)nnz, true, true, false, (gstype)s.nr, (gstype)s.col_shift);

@@ -16,6 +16,9 @@
 //   nsp_amd_run load A.bin nprocs rank out.bin
 //       the same block built on the device (load_MatrixMPI), copied back
 //       and written
+//   nsp_amd_run ascii A.txt nprocs rank out.bin
+//       the same from an ASCII COO file (load_MatrixMPI with ascii set, as
+//       main.cu does for its ascii argument)
 //   nsp_amd_run galerkin A.bin P.bin [C.bin]
 //       the coarse operator P^T (A P) (galerkin_product)
 //   nsp_amd_run mm A.bin out.mtx
@@ -148,26 +151,34 @@ static int run_galerkin(int argc, char **argv) {
   return 0;
 }
 
-static int run_split(int argc, char **argv, bool device) {
+static int run_split(int argc, char **argv, bool device, bool ascii = false) {
   if (argc < 6) return 2;
   const int nprocs = atoi(argv[3]), rank = atoi(argv[4]);
   int64_t nr = 0, nc = 0, nt = 0;
   const auto t0 = std::chrono::steady_clock::now();
   CSR *A = nullptr;
   if (device) {
-    CSR *Ad = load_MatrixMPI(argv[2], nprocs, rank);
+    CSR *Ad = load_MatrixMPI(argv[2], nprocs, rank, 1, ascii);
     const double td = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     A = CSRm::copyToHost(Ad);
     CSRm::free(Ad);
-    nsp_coo_share_t s;
-    if (nsp_coo_share(argv[2], 1, nprocs, rank, &s) != NSP_OK) throw Error(NSP_ERR_INVALID, "share");
-    nr = s.nr;
-    nc = s.nc;
-    nt = s.nt;
+    if (ascii) {
+      nsp_ascii_share_t s;
+      if (nsp_ascii_share(argv[2], nprocs, rank, &s) != NSP_OK) throw Error(NSP_ERR_INVALID, "share");
+      nr = s.nr;
+      nc = s.nc;
+      nt = s.nt;
+    } else {
+      nsp_coo_share_t s;
+      if (nsp_coo_share(argv[2], 1, nprocs, rank, &s) != NSP_OK) throw Error(NSP_ERR_INVALID, "share");
+      nr = s.nr;
+      nc = s.nc;
+      nt = s.nt;
+    }
     write_binary_matrix(argv[5], A, 0);
-    printf("{\"mode\": \"load\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %lu, \"nnz\": %lu, "
+    printf("{\"mode\": \"%s\", \"nr\": %lld, \"nc\": %lld, \"nt\": %lld, \"rows\": %lu, \"nnz\": %lu, "
            "\"row_shift\": %lu, \"read_s\": %.6f}\n",
-           (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, A->row_shift, td);
+           ascii ? "ascii" : "load", (long long)nr, (long long)nc, (long long)nt, A->n, A->nnz, A->row_shift, td);
     CSRm::free(A);
     return 0;
   }
@@ -185,7 +196,8 @@ int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr,
             "usage: %s product A.bin B.bin [C.bin] | local A.bin P.bin nprocs rank [C.bin] | "
-            "split|load A.bin nprocs rank out.bin | galerkin A.bin P.bin [C.bin] | mm A.bin out.mtx\n",
+            "split|load A.bin nprocs rank out.bin | ascii A.txt nprocs rank out.bin | galerkin A.bin P.bin [C.bin] | "
+            "mm A.bin out.mtx\n",
             argv[0]);
     return 2;
   }
@@ -195,6 +207,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[1], "local")) rc = run_local(argc, argv);
     else if (!strcmp(argv[1], "split")) rc = run_split(argc, argv, false);
     else if (!strcmp(argv[1], "load")) rc = run_split(argc, argv, true);
+    else if (!strcmp(argv[1], "ascii")) rc = run_split(argc, argv, true, true);
     else if (!strcmp(argv[1], "galerkin")) rc = run_galerkin(argc, argv);
     else if (!strcmp(argv[1], "mm")) rc = run_mm(argc, argv);
     if (rc == 2) fprintf(stderr, "bad arguments\n");

@@ -15,7 +15,7 @@ per rank, the remainder to the last rank; a rank's rows run from its first
 to its last present row and its columns are shifted by the rows of the ranks
 before it (src/distribute.cu:1094-1105).
 
-  load_MatrixMPI(path, nprocs, rank)   distribute.cu:1071-1125 (binary files,
+  load_MatrixMPI(path, nprocs, rank)   distribute.cu:1071-1125 (binary or ASCII files,
       row partition): the rank's block built on the device.
 """
 from __future__ import annotations
@@ -54,11 +54,15 @@ class CSR:
 
 
 def load_MatrixMPI(path: str, nprocs: int = 1, rank: int = 0, header_type: int = 1,
-                   h: Handle | None = None) -> CSR:
-    """Rank `rank`'s block of a binary COO file as a device CSR (int32 row
-    pointer, FP64), row_shift = the rows of the ranks before it."""
+                   h: Handle | None = None, ascii: bool = False) -> CSR:
+    """Rank `rank`'s block of a binary COO file (or, with `ascii`, of an
+    ASCII one: distribute.cu:1084-1086) as a device CSR (int32 row pointer,
+    FP64), row_shift = the rows of the ranks before it."""
     h = h or default_handle()
-    s, X = h.coo_read(path, nprocs, rank, header_type)
+    if ascii:
+        s, X = h.ascii_read(path, nprocs, rank)
+    else:
+        s, X = h.coo_read(path, nprocs, rank, header_type)
     return CSR.from_device(X, full_n=int(s.nr), row_shift=int(s.col_shift))
 
 

@@ -18,6 +18,7 @@
 // order the reference's stable bubble sort leaves.  One 24-byte read and a
 // 12-byte write per record: HBM-bound.
 #include <fcntl.h>
+#include <stdio.h>
 #include <hip/hip_runtime.h>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -37,7 +38,8 @@ constexpr int kCooBlock = 256;
 
 // Control words of one conversion in device memory.
 struct CooCtl {
-  uint32_t err;          // bit 0: records out of row order or outside the rows; bit 1: column overflow
+  uint32_t err;          // bit 0: records out of row order or outside the rows; bit 1: column overflow;
+                         // bit 2: an ASCII record line not ending in a newline
   uint32_t pad;
   unsigned long long nunsorted;  // rows listed for sorting
 };
@@ -174,21 +176,27 @@ struct CooRun {
   int64_t *list = nullptr;
   CooCtl *ctl = nullptr;
   uint64_t *stage[2] = {nullptr, nullptr};
+  char *text[2] = {nullptr, nullptr};  // ASCII reader: record lines of a chunk
 };
 
-static int coo_begin(Handle *h, int64_t nrows, int64_t stage_recs, CooRun &run) {
+static int coo_begin(Handle *h, int64_t nrows, int64_t stage_recs, CooRun &run, size_t text_bytes = 0) {
   const int64_t nwords = (nrows + 31) / 32;
   const size_t b_bits = al256((size_t)std::max<int64_t>(nwords, 1) * 4);
   const size_t b_list = al256((size_t)std::max<int64_t>(nrows, 1) * 8);
   const size_t b_ctl = 256;
   const size_t b_stage = al256((size_t)stage_recs * 24);
-  NSP_TRY(ws_reserve(h, h->coo, b_bits + b_list + b_ctl + 2 * b_stage, "COO"));
+  const size_t b_text = text_bytes ? al256(text_bytes + 16) : 0;  // + the parser's word-rounded tail
+  NSP_TRY(ws_reserve(h, h->coo, b_bits + b_list + b_ctl + 2 * b_stage + 2 * b_text, "COO"));
   char *p = (char *)h->coo.base;
   run.bits = (uint32_t *)p;
   run.list = (int64_t *)(p + b_bits);
   run.ctl = (CooCtl *)(p + b_bits + b_list);
   run.stage[0] = (uint64_t *)(p + b_bits + b_list + b_ctl);
   run.stage[1] = (uint64_t *)(p + b_bits + b_list + b_ctl + b_stage);
+  if (text_bytes) {
+    run.text[0] = p + b_bits + b_list + b_ctl + 2 * b_stage;
+    run.text[1] = run.text[0] + b_text;
+  }
   hipLaunchKernelGGL(k_coo_init, dim3(grid_of(h, nwords)), dim3(256), 0, h->stream, run.bits, nwords, run.ctl);
   NSP_HIP(h, hipGetLastError());
   return NSP_OK;
@@ -234,6 +242,8 @@ static int coo_finish(Handle *h, const CooRun &run, nsp_csr_t *out) {
   NSP_HIP(h, hipStreamSynchronize(st));
   if (ctl.err & 1)
     return set_error(h, NSP_ERR_INVALID, "COO records out of row order or outside the rank's rows");
+  if (ctl.err & 4)
+    return set_error(h, NSP_ERR_INVALID, "ASCII COO: a record line does not end in a newline (lines of unequal length)");
   if (ctl.err & 2) return set_error(h, NSP_ERR_OVERFLOW, "COO column does not fit int32 after the shift");
   const int64_t n = (int64_t)ctl.nunsorted;
   if (n == 0) return NSP_OK;
@@ -462,6 +472,303 @@ int coo_read(Handle *h, nsp_index_t it, nsp_value_t vt, const char *path, int he
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// ASCII COO files: fixed-length record lines "i j v" parsed on the device,
+// one lane per line, into the 24-byte records the chunk conversion above
+// takes.  The reference reads them with atoi/atol and its own fast_atof
+// (src/READ_MATRIX_MPI.cpp:21-77, 464-477); the parsers below follow the
+// same arithmetic, so the values are bit-identical.
+
+// atol over [p, end): leading white space, an optional sign, decimal digits.
+__host__ __device__ inline int64_t text_atol(const char *p, const char *end) {
+  while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\v' || *p == '\f' || *p == '\r')) p++;
+  bool neg = false;
+  if (p < end && (*p == '-' || *p == '+')) neg = (*p++ == '-');
+  uint64_t v = 0;
+  while (p < end && *p >= '0' && *p <= '9') v = v * 10u + (uint64_t)(*p++ - '0');
+  return neg ? -(int64_t)v : (int64_t)v;
+}
+
+// The reference's decimal parser, on [p, end): blanks (space, tab), an
+// optional sign, integer digits accumulated as value*10 + d, fraction
+// digits added one by one as d / 10^k (10^k built by repeated *10), then
+// an optional exponent (clamped to 308) applied as a product of 1e50, 1e8
+// and 10 factors, divided by for a negative exponent.  Every operation is a
+// separately rounded FP64 operation (no fused multiply-add), as in the
+// reference's host build.
+__host__ __device__ inline double text_fast_atof(const char *p, const char *end) {
+#pragma clang fp contract(off)
+  while (p < end && (*p == ' ' || *p == '\t')) p++;
+  double sign = 1.0;
+  if (p < end && *p == '-') {
+    sign = -1.0;
+    p++;
+  } else if (p < end && *p == '+') {
+    p++;
+  }
+  double value = 0.0;
+  for (; p < end && *p >= '0' && *p <= '9'; p++) value = value * 10.0 + (double)(*p - '0');
+  if (p < end && *p == '.') {
+    double pow10 = 10.0;
+    for (p++; p < end && *p >= '0' && *p <= '9'; p++) {
+      value += (double)(*p - '0') / pow10;
+      pow10 *= 10.0;
+    }
+  }
+  bool neg_exp = false;
+  double scale = 1.0;
+  if (p < end && (*p == 'e' || *p == 'E')) {
+    p++;
+    if (p < end && *p == '-') {
+      neg_exp = true;
+      p++;
+    } else if (p < end && *p == '+') {
+      p++;
+    }
+    uint32_t e = 0;
+    for (; p < end && *p >= '0' && *p <= '9'; p++) e = e * 10u + (uint32_t)(*p - '0');
+    if (e > 308) e = 308;
+    for (; e >= 50; e -= 50) scale *= 1e50;
+    for (; e >= 8; e -= 8) scale *= 1e8;
+    for (; e > 0; e--) scale *= 10.0;
+  }
+  return sign * (neg_exp ? value / scale : value * scale);
+}
+
+// Lines per LDS tile of the parser: at most 16 KB of text, at most one line
+// per thread.
+constexpr int kAsciiTileBytes = 16384;
+
+// m record lines of L bytes each at `text` (the buffer holds whole 4-byte
+// words past the last line) -> 3 words per record {i, j, v}, 1-based as in
+// the file.  A line whose last byte is not a newline sets error bit 2.
+__global__ __launch_bounds__(kCooBlock) void k_ascii_parse(const char *__restrict__ text, int64_t m, int L,
+                                                           int off_col, int off_val, uint64_t *__restrict__ words,
+                                                           CooCtl *ctl) {
+  __shared__ uint32_t tile[kAsciiTileBytes / 4 + 2];
+  const int T = kAsciiTileBytes / L < kCooBlock ? kAsciiTileBytes / L : kCooBlock;  // lines per tile (>= 16)
+  const int64_t ntiles = (m + T - 1) / T;
+  const uint32_t *tw = reinterpret_cast<const uint32_t *>(text);
+  for (int64_t tl = blockIdx.x; tl < ntiles; tl += gridDim.x) {
+    const int64_t l0 = tl * T;
+    const int nl = (int)(m - l0 < T ? m - l0 : T);
+    const int64_t b0 = l0 * L, b1 = b0 + (int64_t)nl * L;
+    const int64_t w0 = b0 >> 2, w1 = (b1 + 3) >> 2;  // words [w0, w1) cover the tile's bytes
+    __syncthreads();  // the previous tile's lines are parsed
+    for (int64_t w = w0 + threadIdx.x; w < w1; w += kCooBlock) tile[w - w0] = tw[w];
+    __syncthreads();
+    if (threadIdx.x < nl) {
+      const char *line = reinterpret_cast<const char *>(tile) + (b0 - 4 * w0) + (int64_t)threadIdx.x * L;
+      const char *end = line + L;
+      const int64_t i = text_atol(line, end);
+      const int64_t j = text_atol(line + off_col, end);
+      const double v = text_fast_atof(line + off_val, end);
+      if (line[L - 1] != '\n') atomicOr(&ctl->err, 4u);
+      const int64_t q = l0 + threadIdx.x;
+      words[3 * q] = (uint64_t)i;
+      words[3 * q + 1] = (uint64_t)j;
+      words[3 * q + 2] = (uint64_t)__double_as_longlong(v);
+    }
+  }
+}
+
+// Header and first record line: body offset, line length and the two field
+// offsets, found as the reference finds them (src/READ_MATRIX_MPI.cpp:
+// 381-429), except that the body starts after the '%' lines too.
+static int ascii_layout(FILE *f, nsp_ascii_share_t &s) {
+  char *line = nullptr;
+  size_t cap = 0;
+  int64_t off = 0;
+  ssize_t n;
+  int rc = NSP_ERR_INVALID;
+  while ((n = getline(&line, &cap, f)) > 0 && line[0] == '%') off += n;
+  if (n > 0) {
+    long a = 0, b = 0, c = 0;
+    const int k = sscanf(line, "%ld %ld %ld", &a, &b, &c);
+    bool ok = true;
+    if (k == 3) {
+      s.nr = a;
+      s.nc = b;
+      s.nt = c;
+    } else if (sscanf(line, "%ld %ld", &a, &b) == 2) {
+      s.nr = s.nc = a;
+      s.nt = b;
+    } else {
+      ok = false;
+    }
+    off += n;
+    s.body_offset = off;
+    if (ok && s.nr >= 0 && s.nt >= 0) {
+      if (s.nt == 0) {
+        s.line_bytes = 1;
+        s.col_offset = s.val_offset = 0;
+        rc = NSP_OK;
+      } else if ((n = getline(&line, &cap, f)) > 1) {
+        // first space after the row field, then after the column field
+        ssize_t i = 0;
+        while (i < n && line[i] == ' ') i++;
+        while (i < n && line[i] != ' ' && line[i] != '\n') i++;
+        s.col_offset = (int32_t)i;
+        while (i < n && line[i] == ' ') i++;
+        while (i < n && line[i] != ' ' && line[i] != '\n') i++;
+        s.val_offset = (int32_t)i;
+        s.line_bytes = line[n - 1] == '\n' ? n : n + 1;
+        rc = NSP_OK;
+      }
+    }
+  }
+  free(line);
+  return rc;
+}
+
+// Row field of record q (the reference's binsearch_iat_str reads it with atol).
+static bool ascii_row(int fd, const nsp_ascii_share_t &s, int64_t q, int64_t &row) {
+  char buf[64];
+  const size_t want = (size_t)std::min<int64_t>(s.line_bytes, (int64_t)sizeof buf);
+  const ssize_t got = pread(fd, buf, want, (off_t)(s.body_offset + q * s.line_bytes));
+  if (got <= 0) return false;
+  row = text_atol(buf, buf + got);
+  return true;
+}
+
+// First record whose row exceeds x (src/READ_MATRIX_MPI.cpp:83-104).
+static int64_t ascii_upper_row(int fd, const nsp_ascii_share_t &s, int64_t x, bool &ok) {
+  int64_t lo = 0, hi = s.nt;
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    int64_t row;
+    if (!ascii_row(fd, s, mid, row)) {
+      ok = false;
+      return 0;
+    }
+    if (x < row) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+static int ascii_share_of(int fd, const nsp_ascii_share_t &s, int nprocs, int rank, int64_t &b, int64_t &e,
+                          int64_t &first_row, int64_t &nrows) {
+  const int64_t per = s.nr / nprocs;
+  const int64_t lo = per * rank, hi = (rank == nprocs - 1) ? s.nr : per * (rank + 1);
+  bool ok = true;
+  b = ascii_upper_row(fd, s, lo, ok);
+  e = ascii_upper_row(fd, s, hi, ok);
+  if (!ok) return NSP_ERR_INVALID;
+  first_row = lo;
+  nrows = 0;
+  if (e > b) {
+    int64_t r0, r1;
+    if (!ascii_row(fd, s, b, r0) || !ascii_row(fd, s, e - 1, r1)) return NSP_ERR_INVALID;
+    first_row = r0 - 1;
+    nrows = r1 - r0 + 1;
+  }
+  return NSP_OK;
+}
+
+int ascii_share(const char *path, int nprocs, int rank, nsp_ascii_share_t *out) {
+  if (!path || !out || nprocs < 1 || rank < 0 || rank >= nprocs) return NSP_ERR_INVALID;
+  FILE *f = fopen(path, "rb");
+  if (!f) return NSP_ERR_INVALID;
+  nsp_ascii_share_t s{};
+  int rc = ascii_layout(f, s);
+  struct stat sb;
+  // nt lines of line_bytes each; the last one may end at the end of the file
+  if (rc == NSP_OK && (fstat(fileno(f), &sb) != 0 ||
+                       s.body_offset + s.nt * s.line_bytes - (s.nt > 0 ? 1 : 0) > (int64_t)sb.st_size))
+    rc = NSP_ERR_INVALID;
+  const int fd = fileno(f);
+  if (rc == NSP_OK) rc = ascii_share_of(fd, s, nprocs, rank, s.rec_begin, s.rec_end, s.first_row, s.nrows);
+  for (int r = 0; r < rank && rc == NSP_OK; r++) {  // col_shift: MPI_Exscan of rowxproc
+    int64_t b, e, fr, n;
+    rc = ascii_share_of(fd, s, nprocs, r, b, e, fr, n);
+    s.col_shift += n;
+  }
+  fclose(f);
+  if (rc == NSP_OK) *out = s;
+  return rc;
+}
+
+template <typename RowT, typename V>
+static int ascii_read_impl(Handle *h, int fd, int64_t file_bytes, const nsp_ascii_share_t *s, nsp_csr_t *out) {
+  const int64_t nrec = s->rec_end - s->rec_begin, nrows = s->nrows, L = s->line_bytes;
+  hipStream_t st = h->stream;
+  if (nrec == 0) {
+    hipLaunchKernelGGL((k_coo_fill_rpt<RowT>), dim3(grid_of(h, nrows + 1)), dim3(256), 0, st, (RowT *)out->row_ptr,
+                       nrows + 1);
+    NSP_HIP(h, hipGetLastError());
+    NSP_HIP(h, hipStreamSynchronize(st));
+    return NSP_OK;
+  }
+  // lines per streamed chunk: the records' chunk, held to 256 MiB of text
+  const int64_t chunk = std::max<int64_t>(1, std::min({h->coo_chunk, nrec, (int64_t(1) << 28) / L}));
+  CooRun run;
+  NSP_TRY(coo_begin(h, nrows, chunk, run, (size_t)(chunk * L)));
+  NSP_TRY(pin_reserve(h, (size_t)(chunk * L) * 2));
+  for (int k = 0; k < 2; k++)
+    if (!h->pin_ev[k]) NSP_HIP(h, hipEventCreateWithFlags(&h->pin_ev[k], hipEventDisableTiming));
+  const int nthreads = (int)std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const int T = std::min<int>(kAsciiTileBytes / (int)L, kCooBlock);
+  int64_t prev_i = 0, prev_j = 0;
+  int k = 0;
+  for (int64_t q0 = 0; q0 < nrec; q0 += chunk, k++) {
+    const int64_t m = std::min(chunk, nrec - q0);
+    char *hbuf = (char *)h->pin + (size_t)(k & 1) * (size_t)(chunk * L);
+    if (k >= 2) NSP_HIP(h, hipEventSynchronize(h->pin_ev[k & 1]));
+    const int64_t off = s->body_offset + (s->rec_begin + q0) * L;
+    // the file's last line may lack its newline: read what is there and
+    // supply it
+    const int64_t avail = std::min<int64_t>(m * L, file_bytes - off);
+    if (avail < m * L - 1 || !read_parallel(fd, hbuf, (size_t)avail, off, nthreads))
+      return set_error(h, NSP_ERR_INVALID, "ASCII COO: short read");
+    if (avail < m * L) hbuf[m * L - 1] = '\n';
+    NSP_HIP(h, hipMemcpyAsync(run.text[k & 1], hbuf, (size_t)(m * L), hipMemcpyHostToDevice, st));
+    NSP_HIP(h, hipEventRecord(h->pin_ev[k & 1], st));
+    if (k == 0) coo_mark(h, 0);
+    const int g = grid_of(h, m, T);
+    hipLaunchKernelGGL(k_ascii_parse, dim3(g), dim3(kCooBlock), 0, st, run.text[k & 1], m, (int)L, s->col_offset,
+                       s->val_offset, run.stage[0], run.ctl);
+    NSP_HIP(h, hipGetLastError());
+    NSP_TRY((coo_launch_chunk<RowT, V>(h, run, run.stage[0], m, q0, prev_i, prev_j, s->first_row + 1, nrows, nrec,
+                                       s->col_shift, out)));
+    const char *last = hbuf + (m - 1) * L;
+    prev_i = text_atol(last, last + L);
+    prev_j = text_atol(last + s->col_offset, last + L);
+  }
+  coo_mark(h, 1);
+  NSP_TRY((coo_finish<RowT, V>(h, run, out)));
+  coo_collect(h);
+  return NSP_OK;
+}
+
+int ascii_read(Handle *h, nsp_index_t it, nsp_value_t vt, const char *path, const nsp_ascii_share_t *s,
+               nsp_csr_t *out) {
+  if (!path || !s) return set_error(h, NSP_ERR_INVALID, "ASCII COO: bad arguments");
+  const int64_t nrec = s->rec_end - s->rec_begin;
+  if (out) out->rows = s->nrows;
+  NSP_TRY(coo_check_args(h, it, nrec, s->nrows, out));
+  if (s->rec_begin < 0 || nrec < 0 || s->line_bytes < 2 || s->line_bytes > kAsciiTileBytes / 16 ||
+      s->col_offset < 0 || s->val_offset < s->col_offset || s->val_offset >= s->line_bytes)
+    return set_error(h, NSP_ERR_INVALID, "ASCII COO: bad share (record lines of 2 to 1024 bytes)");
+  out->cols = s->nc;
+  out->nnz = nrec;
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) return set_error(h, NSP_ERR_INVALID, std::string("ASCII COO: cannot open ") + path);
+  struct stat sb;
+  if (fstat(fd, &sb) != 0) {
+    close(fd);
+    return set_error(h, NSP_ERR_INVALID, "ASCII COO: cannot stat the file");
+  }
+  int rc;
+  if (it == NSP_IDX32 && vt == NSP_F64) rc = ascii_read_impl<int32_t, double>(h, fd, (int64_t)sb.st_size, s, out);
+  else if (it == NSP_IDX32) rc = ascii_read_impl<int32_t, float>(h, fd, (int64_t)sb.st_size, s, out);
+  else if (vt == NSP_F64) rc = ascii_read_impl<int64_t, double>(h, fd, (int64_t)sb.st_size, s, out);
+  else rc = ascii_read_impl<int64_t, float>(h, fd, (int64_t)sb.st_size, s, out);
+  if (rc != NSP_OK) (void)hipStreamSynchronize(h->stream);  // no copy may still read the pinned buffers
+  close(fd);
+  return rc;
+}
+
 }  // namespace nsp
 
 extern "C" {
@@ -480,6 +787,16 @@ int nsp_coo_read(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const char *pa
                  const nsp_coo_share_t *share, nsp_csr_t *out) {
   if (!hs) return NSP_ERR_INVALID;
   return nsp::coo_read(nsp::handle_of(hs), it, vt, path, header_type, share, out);
+}
+
+int nsp_ascii_share(const char *path, int nprocs, int rank, nsp_ascii_share_t *out) {
+  return nsp::ascii_share(path, nprocs, rank, out);
+}
+
+int nsp_ascii_read(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const char *path, const nsp_ascii_share_t *share,
+                   nsp_csr_t *out) {
+  if (!hs) return NSP_ERR_INVALID;
+  return nsp::ascii_read(nsp::handle_of(hs), it, vt, path, share, out);
 }
 
 }  // extern "C"

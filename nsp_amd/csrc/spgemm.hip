@@ -743,14 +743,34 @@ __global__ __launch_bounds__(256) void k_sym_narrow(int bin_lo, int bin_hi, int6
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
   uint32_t *Bm = bits[slot];
   const XcdTiles xt((count + RPB - 1) / RPB);
-  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
+  // the group's rows, one tile apart; the next row's A extent and first
+  // chunk of entries are loaded while the current row is counted
+  auto row_at = [&](int64_t tile) -> int32_t {
     const int64_t t = tile * RPB + slot;
-    if (t < count) {
-      const int32_t r = perm ? perm[off + t] : (int32_t)t;
+    return (tile < xt.end && t < count) ? (perm ? perm[off + t] : (int32_t)t) : -1;
+  };
+  int32_t rn = row_at(xt.first);
+  int64_t an0 = 0, an1 = 0;
+  FirstChunk<float> pre;
+  if (rn >= 0) {
+    an0 = arpt[rn];
+    an1 = arpt[rn + 1];
+    pre = load_first_chunk<false, float>(glane, an0, an1, ab_b0, ab_len, nullptr);
+  }
+  for (int64_t tile = xt.first; rn >= 0; tile += xt.step) {
+    {
+      const int32_t r = rn;
+      const int64_t a0 = an0, a1 = an1;
+      const FirstChunk<float> cur = pre;
+      rn = row_at(tile + xt.step);
+      if (rn >= 0) {
+        an0 = arpt[rn];
+        an1 = arpt[rn + 1];
+      }
       clear_lds<GS, kNarrowWords>(Bm, glane, 0);
       wave_lds_sync();
       row_products<GS, R, false, float, false, O32>(
-          glane, arpt[r], arpt[r + 1], 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[slot],
+          glane, a0, a1, 0, GS, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[slot],
           [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
 #pragma unroll
             for (int u = 0; u < R; u++) {
@@ -758,7 +778,9 @@ __global__ __launch_bounds__(256) void k_sym_narrow(int bin_lo, int bin_hi, int6
               const uint32_t k = (uint32_t)key[u] & (kNarrowCols - 1);
               atomicOr(&Bm[ok[u] ? (k >> 5) : (uint32_t)glane], ok[u] ? (1u << (k & 31)) : 0u);
             }
-          });
+          },
+          &cur);
+      if (rn >= 0) pre = load_first_chunk<false, float>(glane, an0, an1, ab_b0, ab_len, nullptr);
       wave_lds_sync();
       int cnt = 0;
       if constexpr (WPL >= 4) {
@@ -824,18 +846,40 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
     }
   };
   const XcdTiles xt((count + RPB - 1) / RPB);
-  for (int64_t tile = xt.first; tile < xt.end; tile += xt.step) {
+  // the group's rows, one tile apart; the next row's A extent and first
+  // chunk of entries are loaded while the current row is accumulated
+  auto row_at = [&](int64_t tile) -> int32_t {
     const int64_t t = tile * RPB + slot;
-    if (t < count) {
-      const int32_t r = perm ? perm[off + t] : (int32_t)t;
-      if (crpt[r + 1] > info->cap) {  // C buffers too small: skip the row, flag it
+    return (tile < xt.end && t < count) ? (perm ? perm[off + t] : (int32_t)t) : -1;
+  };
+  int32_t rn = row_at(xt.first);
+  int64_t an0 = 0, an1 = 0;
+  FirstChunk<V> pre;
+  if (rn >= 0) {
+    an0 = arpt[rn];
+    an1 = arpt[rn + 1];
+    pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
+  }
+  for (int64_t tile = xt.first; rn >= 0; tile += xt.step) {
+    {
+      const int32_t r = rn;
+      const int64_t a0 = an0, a1 = an1;
+      const FirstChunk<V> cur = pre;
+      rn = row_at(tile + xt.step);
+      if (rn >= 0) {
+        an0 = arpt[rn];
+        an1 = arpt[rn + 1];
+      }
+      const int64_t c0 = crpt[r];
+      const int nz = (int)(crpt[r + 1] - c0);
+      if (c0 + nz > info->cap) {  // C buffers too small: skip the row, flag it
         if (glane == 0) atomicOr((unsigned int *)&const_cast<Info *>(info)->error, (unsigned int)NSP_ERR_OVERFLOW);
+        if (rn >= 0) pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
         continue;
       }
       clear_lds<GS, kNarrowWords>(Wb, glane, 0);
       clear_lds<GS, NS + GS>(Vs, glane, 0);
       wave_lds_sync();
-      const int64_t a0 = arpt[r], a1 = arpt[r + 1];
       // walk 1: the row's columns into the bitmap; a row that is one window
       // long keeps its products in registers for the accumulation, with a
       // flag on the product that set its column's bit first
@@ -858,7 +902,9 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
               sf[u] = ok[u] && !(old & bit);
             }
             windows++;
-          });
+          },
+          &cur);
+      if (rn >= 0) pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
       wave_lds_sync();
       // running counts: lane g holds words [g*WPL, (g+1)*WPL)
       {
@@ -877,8 +923,6 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
         }
       }
       wave_lds_sync();
-      const int64_t c0 = crpt[r];
-      const int nz = (int)(crpt[r + 1] - c0);
       if (windows == 1) {
         accumulate(sk, sp, so, sf, ccol + c0);
       } else {
@@ -887,7 +931,8 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
             glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
             [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
               accumulate(key, prod, ok, nullptr, nullptr);
-            });
+            },
+            &cur);
         // the columns from the bits (positions from the running counts)
 #pragma unroll
         for (int w = 0; w < WPL; w++) {

@@ -388,6 +388,35 @@ void orc_backward(int64_t M, const int64_t *arpt, const int32_t *acol, const dou
  * caller (free with orc_free). */
 typedef struct { int64_t i, j; double v; } orc_coo_rec;
 
+/* The rank's n records (1-based, sorted by row) as its CSR block: rows from
+ * the first to the last present row (distribute.cu:1094-1096; `first` when
+ * there are none), columns 0-based minus col_shift (1103), rows with
+ * unsorted columns sorted stably (check_and_fix_order, 1050-1069). */
+static void coo_records_csr(const orc_coo_rec *rec, int64_t n, int64_t first, int64_t col_shift,
+                            int64_t *out_nloc, int64_t **out_rpt, int32_t **out_col, double **out_val) {
+    int64_t r_first = n > 0 ? rec[0].i - 1 : first;
+    int64_t r_last = n > 0 ? rec[n - 1].i - 1 : first - 1;
+    int64_t nloc = r_last - r_first + 1;
+    int64_t *rpt = (int64_t *)calloc((size_t)nloc + 1, sizeof(int64_t));
+    int32_t *col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    double *val = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int64_t q = 0; q < n; q++) rpt[rec[q].i - 1 - r_first + 1]++;
+    for (int64_t r = 0; r < nloc; r++) rpt[r + 1] += rpt[r];
+    for (int64_t q = 0; q < n; q++) {
+        col[q] = (int32_t)(rec[q].j - 1 - col_shift);
+        val[q] = rec[q].v;
+    }
+    for (int64_t r = 0; r < nloc; r++) {
+        for (int64_t p = rpt[r] + 1; p < rpt[r + 1]; p++) {
+            int32_t c = col[p]; double v = val[p];
+            int64_t q = p - 1;
+            while (q >= rpt[r] && col[q] > c) { col[q + 1] = col[q]; val[q + 1] = val[q]; q--; }
+            col[q + 1] = c; val[q + 1] = v;
+        }
+    }
+    *out_nloc = nloc; *out_rpt = rpt; *out_col = col; *out_val = val;
+}
+
 static int64_t coo_lower_bound(FILE *f, int64_t hdr, int64_t nt, int64_t row1) {
     /* first record whose 1-based row id is >= row1 */
     int64_t lo = 0, hi = nt;
@@ -437,36 +466,145 @@ int orc_read_binary_coo(const char *path, int header_type, int nprocs, int rank,
     fseeko(f, (off_t)(hdr + b * (int64_t)sizeof(orc_coo_rec)), SEEK_SET);
     if (n > 0 && fread(rec, sizeof(orc_coo_rec), (size_t)n, f) != (size_t)n) { free(rec); fclose(f); return 4; }
     fclose(f);
-    /* rows actually present define the local row range (distribute.cu:1096) */
-    int64_t r_first = n > 0 ? rec[0].i - 1 : first;
-    int64_t r_last = n > 0 ? rec[n - 1].i - 1 : first - 1;
-    int64_t nloc = r_last - r_first + 1;
-    int64_t *rpt = (int64_t *)calloc((size_t)nloc + 1, sizeof(int64_t));
-    int32_t *col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-    double *val = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
-    for (int64_t q = 0; q < n; q++) rpt[rec[q].i - 1 - r_first + 1]++;
-    for (int64_t r = 0; r < nloc; r++) rpt[r + 1] += rpt[r];
-    for (int64_t q = 0; q < n; q++) {
-        col[q] = (int32_t)(rec[q].j - 1 - col_shift);
-        val[q] = rec[q].v;
-    }
+    coo_records_csr(rec, n, first, col_shift, out_nloc, out_rpt, out_col, out_val);
     free(rec);
-    /* check_and_fix_order: stable insertion sort of rows found unsorted */
-    for (int64_t r = 0; r < nloc; r++) {
-        for (int64_t p = rpt[r] + 1; p < rpt[r + 1]; p++) {
-            int32_t c = col[p]; double v = val[p];
-            int64_t q = p - 1;
-            while (q >= rpt[r] && col[q] > c) { col[q + 1] = col[q]; val[q + 1] = val[q]; q--; }
-            col[q + 1] = c; val[q + 1] = v;
-        }
-    }
     *out_nr = nr; *out_nc = nc; *out_nt = nt;
-    *out_nloc = nloc; *out_row_shift = col_shift; *out_nnz = n;
-    *out_rpt = rpt; *out_col = col; *out_val = val;
+    *out_row_shift = col_shift; *out_nnz = n;
     return 0;
 }
 
 void orc_free(void *p) { free(p); }
+
+/* ---- ASCII COO files (READ_ASCII_MATRIX_MPI, src/READ_MATRIX_MPI.cpp:364-578) ----
+ * Text file: lines starting with '%', a header "nr nc nt" or "nr nt" (nc =
+ * nr, 381-388), then nt record lines "i j v" of the first record line's
+ * length (row_size, 414-429), 1-based and sorted by row.  The column field
+ * is read at off_1 and the value at off_2, the first space after the first
+ * and second fields of the first record line (419-424).  Rows are split as
+ * for the binary files (440-455, binsearch_iat_str 83-104), rows and
+ * columns read with atol (column - 1, 474-477), values with the
+ * reference's fast_atof (21-77).  The body starts after the '%' lines and
+ * the header (the reference adds only the header line's length, 412-413,
+ * which is the same offset for a file without '%' lines). */
+
+/* atol over [p, end) */
+static int64_t orc_atol(const char *p, const char *end) {
+    while (p < end && (*p == ' ' || *p == '\t' || *p == '\n' || *p == '\v' || *p == '\f' || *p == '\r')) p++;
+    int neg = 0;
+    if (p < end && (*p == '-' || *p == '+')) neg = (*p++ == '-');
+    uint64_t v = 0;
+    while (p < end && *p >= '0' && *p <= '9') v = v * 10u + (uint64_t)(*p++ - '0');
+    return neg ? -(int64_t)v : (int64_t)v;
+}
+
+/* fast_atof (READ_MATRIX_MPI.cpp:21-77): blanks, sign, integer digits as
+ * value*10 + d, fraction digits as value += d / pow10 with pow10 *= 10,
+ * exponent (clamped to 308) as a product of 1e50, 1e8 and 10 factors. */
+double orc_fast_atof(const char *p, const char *end) {
+    while (p < end && (*p == ' ' || *p == '\t')) p++;
+    double sign = 1.0;
+    if (p < end && *p == '-') { sign = -1.0; p++; }
+    else if (p < end && *p == '+') p++;
+    double value = 0.0;
+    for (; p < end && *p >= '0' && *p <= '9'; p++) value = value * 10.0 + (double)(*p - '0');
+    if (p < end && *p == '.') {
+        double pow10 = 10.0;
+        for (p++; p < end && *p >= '0' && *p <= '9'; p++) { value += (double)(*p - '0') / pow10; pow10 *= 10.0; }
+    }
+    int frac = 0;
+    double scale = 1.0;
+    if (p < end && (*p == 'e' || *p == 'E')) {
+        p++;
+        if (p < end && *p == '-') { frac = 1; p++; }
+        else if (p < end && *p == '+') p++;
+        unsigned int e = 0;
+        for (; p < end && *p >= '0' && *p <= '9'; p++) e = e * 10u + (unsigned int)(*p - '0');
+        if (e > 308) e = 308;
+        while (e >= 50) { scale *= 1e50; e -= 50; }
+        while (e >= 8) { scale *= 1e8; e -= 8; }
+        while (e > 0) { scale *= 10.0; e -= 1; }
+    }
+    return sign * (frac ? (value / scale) : (value * scale));
+}
+
+static int64_t ascii_upper(const char *body, int64_t L, int64_t nt, int64_t x) {
+    /* first record line whose row exceeds x */
+    int64_t lo = 0, hi = nt;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (x < orc_atol(body + mid * L, body + (mid + 1) * L)) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+int orc_read_ascii_coo(const char *path, int nprocs, int rank,
+                       int64_t *out_nr, int64_t *out_nc, int64_t *out_nt,
+                       int64_t *out_nloc, int64_t *out_row_shift, int64_t *out_nnz,
+                       int64_t **out_rpt, int32_t **out_col, double **out_val) {
+    FILE *f = fopen(path, "rb");
+    if (!f) return 1;
+    fseeko(f, 0, SEEK_END);
+    int64_t size = (int64_t)ftello(f);
+    fseeko(f, 0, SEEK_SET);
+    char *buf = (char *)malloc((size_t)size + 2);
+    if (fread(buf, 1, (size_t)size, f) != (size_t)size) { free(buf); fclose(f); return 2; }
+    fclose(f);
+    buf[size] = '\n';  /* the last line may lack its newline */
+    buf[size + 1] = 0;
+    int64_t off = 0;
+    while (off < size && buf[off] == '%') { while (off < size && buf[off] != '\n') off++; off++; }
+    if (off >= size) { free(buf); return 2; }
+    long a = 0, b = 0, c = 0;
+    int64_t nr, nc, nt;
+    char hdr[1025];
+    int64_t hl = 0;
+    while (off + hl < size && buf[off + hl] != '\n' && hl < 1024) { hdr[hl] = buf[off + hl]; hl++; }
+    hdr[hl] = 0;
+    if (sscanf(hdr, "%ld %ld %ld", &a, &b, &c) == 3) { nr = a; nc = b; nt = c; }
+    else if (sscanf(hdr, "%ld %ld", &a, &b) == 2) { nr = nc = a; nt = b; }
+    else { free(buf); return 2; }
+    const char *body = buf + off + hl + 1;
+    int64_t body_bytes = size - (off + hl + 1);
+    int64_t L = 1, off1 = 0, off2 = 0;
+    if (nt > 0) {
+        int64_t i = 0;
+        while (i < body_bytes && body[i] == ' ') i++;
+        while (i < body_bytes && body[i] != ' ' && body[i] != '\n') i++;
+        off1 = i;
+        while (i < body_bytes && body[i] == ' ') i++;
+        while (i < body_bytes && body[i] != ' ' && body[i] != '\n') i++;
+        off2 = i;
+        while (i < body_bytes && body[i] != '\n') i++;
+        L = i + 1;
+        if (L < 2 || nt * L - 1 > body_bytes) { free(buf); return 3; }
+    }
+    int64_t per = nr / nprocs;
+    int64_t first = per * rank, last = (rank == nprocs - 1) ? nr : per * (rank + 1);
+    int64_t bq = ascii_upper(body, L, nt, first), eq = ascii_upper(body, L, nt, last);
+    int64_t col_shift = 0;
+    for (int r = 0; r < rank; r++) {
+        int64_t rb = ascii_upper(body, L, nt, per * r), re = ascii_upper(body, L, nt, per * (r + 1));
+        if (re > rb)
+            col_shift += orc_atol(body + (re - 1) * L, body + re * L) - orc_atol(body + rb * L, body + (rb + 1) * L) + 1;
+    }
+    int64_t n = eq - bq;
+    orc_coo_rec *rec = (orc_coo_rec *)malloc(sizeof(orc_coo_rec) * (size_t)(n > 0 ? n : 1));
+    for (int64_t q = 0; q < n; q++) {
+        const char *line = body + (bq + q) * L, *end = line + L;
+        if (line[L - 1] != '\n') { free(rec); free(buf); return 4; }  /* lines of unequal length */
+        rec[q].i = orc_atol(line, end);
+        rec[q].j = orc_atol(line + off1, end);
+        rec[q].v = orc_fast_atof(line + off2, end);
+    }
+    free(buf);
+    for (int64_t q = 1; q < n; q++)
+        if (rec[q].i < rec[q - 1].i) { free(rec); return 5; }
+    coo_records_csr(rec, n, first, col_shift, out_nloc, out_rpt, out_col, out_val);
+    free(rec);
+    *out_nr = nr; *out_nc = nc; *out_nt = nt;
+    *out_row_shift = col_shift; *out_nnz = n;
+    return 0;
+}
 
 /* Binary COO writer in the same layout (main.cu:468-512). */
 int orc_write_binary_coo(const char *path, int header_type, int64_t nr, int64_t nc,

@@ -73,6 +73,11 @@ def lib():
         L.orc_read_binary_coo.restype = C.c_int
         L.orc_read_binary_coo.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int] + \
             [C.POINTER(C.c_int64)] * 6 + [C.POINTER(VP)] * 3
+        L.orc_read_ascii_coo.restype = C.c_int
+        L.orc_read_ascii_coo.argtypes = [C.c_char_p, C.c_int, C.c_int] + [C.POINTER(C.c_int64)] * 6 + \
+            [C.POINTER(C.c_void_p)] * 3
+        L.orc_fast_atof.restype = C.c_double
+        L.orc_fast_atof.argtypes = [C.c_char_p, C.c_char_p]
         L.orc_write_binary_coo.restype = C.c_int
         L.orc_write_binary_coo.argtypes = [C.c_char_p, C.c_int, I64, I64, _i64p, _i32p, _f64p, I64]
         L.orc_free.argtypes = [VP]
@@ -238,6 +243,33 @@ def read_binary_coo(path, nprocs=1, rank=0, header_type=1):
     for p in ptrs:
         lib().orc_free(p)
     return nr, nc, nt, shift, HostCSR(rpt, col, val, nloc, nc)
+
+
+def read_ascii_coo(path, nprocs=1, rank=0):
+    """Rank `rank`'s CSR block of an ASCII COO file: (nr, nc, nt, row_shift,
+    HostCSR), as load_MatrixMPI returns it with `ascii` set."""
+    from nsp_amd.gen import HostCSR
+    vals = [C.c_int64() for _ in range(6)]
+    ptrs = [C.c_void_p() for _ in range(3)]
+    rc = lib().orc_read_ascii_coo(path.encode(), nprocs, rank, *[C.byref(v) for v in vals],
+                                  *[C.byref(p) for p in ptrs])
+    if rc != 0:
+        raise IOError(f"orc_read_ascii_coo failed ({rc}) on {path}")
+    nr, nc, nt, nloc, shift, nnz = (v.value for v in vals)
+    rpt = np.ctypeslib.as_array(C.cast(ptrs[0], C.POINTER(C.c_int64)), (nloc + 1,)).copy()
+    col = np.ctypeslib.as_array(C.cast(ptrs[1], C.POINTER(C.c_int32)), (max(nnz, 1),))[:nnz].copy()
+    val = np.ctypeslib.as_array(C.cast(ptrs[2], C.POINTER(C.c_double)), (max(nnz, 1),))[:nnz].copy()
+    for p in ptrs:
+        lib().orc_free(p)
+    return nr, nc, nt, shift, HostCSR(rpt, col, val, nloc, nc)
+
+
+def fast_atof(text: str) -> float:
+    """The reference's decimal parser on one field."""
+    b = text.encode()
+    buf = C.create_string_buffer(b, len(b))
+    end = C.cast(C.addressof(buf) + len(b), C.c_char_p)
+    return lib().orc_fast_atof(buf, end)
 
 
 def write_binary_coo(path, X, header_type=1, col_shift=0):

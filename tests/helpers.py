@@ -36,3 +36,29 @@ def check_against_oracle(A, B, C, dtype="f64", oracle=None, ref=None):
     comp, fro = oracle.componentwise_error(got_val, cval, cabs)
     assert comp <= TOL[dtype], f"componentwise error {comp:.3e} > {TOL[dtype]:.0e} ({dtype})"
     return comp, fro
+
+
+def write_ascii_coo(path, X=None, records=None, nr=None, nc=None, header_fields=3, comments=(),
+                    value_fmt="{:.17g}", final_newline=True, col_shift=0):
+    """ASCII COO file of the reference's text format: '%' comment lines, a
+    header "nr nc nt" (or "nr nt"), then fixed-length record lines "i j v"
+    (1-based; right-aligned row and column fields, the value left-aligned
+    and padded), so every line has the first line's field offsets.  From a
+    HostCSR X (global column = col + col_shift) or a list of (i, j, v)
+    records with 1-based indices (v a float, or a string written as is)."""
+    if records is None:
+        rows = np.repeat(np.arange(X.nrows), np.diff(X.rpt))
+        records = [(int(i) + 1, int(j) + col_shift + 1, float(v)) for i, j, v in zip(rows, X.col, X.val)]
+        nr, nc = X.nrows, X.ncols + col_shift
+    texts = [v if isinstance(v, str) else value_fmt.format(v) for _, _, v in records]
+    w1 = len(str(max([nr] + [i for i, _, _ in records]))) + 1
+    w2 = len(str(max([nc] + [j for _, j, _ in records]))) + 1
+    w3 = max([len(t) for t in texts] + [1])
+    lines = [f"{i:>{w1}d} {j:>{w2}d} {t:<{w3}s}\n" for (i, j, _), t in zip(records, texts)]
+    head = f"{nr} {nc} {len(records)}\n" if header_fields == 3 else f"{nr} {len(records)}\n"
+    body = "".join(lines)
+    if not final_newline and body:
+        body = body[:-1]
+    with open(path, "w") as f:
+        f.write("".join(c + "\n" for c in comments) + head + body)
+    return records

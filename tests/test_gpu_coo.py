@@ -194,3 +194,84 @@ def test_transpose_long_columns(h, index, dtype):
     np.testing.assert_array_equal(XT.col.cpu().numpy(), np.repeat(np.arange(n), np.diff(X.rpt))[order])
     want = X.val[order].astype(np.float32) if dtype == "f32" else X.val[order]
     np.testing.assert_array_equal(XT.val.cpu().numpy(), want)
+
+
+# ---------------------------------------------------------------- ASCII files
+def ascii_case(tmp_path, case, value_fmt):
+    """Text files of the reference's ASCII format (fixed-length record
+    lines): the 12^3 stencil, the mixed records (empty rows, unsorted rows,
+    a repeated column) and the long shuffled rows, values written with the
+    given format (17 significant digits, 7 in exponent form, 3 decimals)."""
+    from tests.helpers import write_ascii_coo
+    if case == "stencil":
+        X = gen.stencil_27pt(12, seed=3)
+        rec, nr, nc = records_of(X), X.nrows, X.ncols
+    elif case == "mixed":
+        rec, nr, nc = mixed_records()
+    else:
+        rec, nr, nc = long_rows_records()
+    p = str(tmp_path / f"{case}.txt")
+    write_ascii_coo(p, records=[(int(i), int(j), float(v)) for i, j, v in rec], nr=nr, nc=nc, value_fmt=value_fmt,
+                    comments=["% written by the tests"] if case == "mixed" else ())
+    return p
+
+
+@pytest.mark.parametrize("case", ["stencil", "mixed", "long_rows"])
+@pytest.mark.parametrize("value_fmt", ["{:.17g}", "{:.6e}", "{:+.3f}"])
+@pytest.mark.parametrize("index,dtype", [("i32", "f64"), ("i64", "f32")])
+def test_ascii_read_matches_oracle(h, orc, tmp_path, case, value_fmt, index, dtype):
+    """nsp_ascii_read against the oracle's restatement of
+    READ_ASCII_MATRIX_MPI + load_MatrixMPI: row pointer and columns
+    bit-exact, values bit-exact (the device parses with the reference's
+    fast_atof steps), for 1-4 ranks and streamed chunks down to one line."""
+    import nsp_amd
+    p = ascii_case(tmp_path, case, value_fmt)
+    chunks = {"stencil": (1 << 21, 1000), "mixed": (1 << 21, 7, 1), "long_rows": (1 << 21, 4099)}[case]
+    for nprocs in ((1, 4) if case == "stencil" else (1, 2, 3, 4)):
+        for rank in range(nprocs):
+            nr, nc, nt, shift, ref = orc.read_ascii_coo(p, nprocs, rank)
+            s = nsp_amd.ascii_share(p, nprocs, rank)
+            assert (s.nr, s.nc, s.nt, s.col_shift) == (nr, nc, nt, shift)
+            for chunk in chunks:
+                print(f"  ascii {case} {value_fmt} {nprocs}:{rank} chunk={chunk} {index}/{dtype}", flush=True)
+                h.set_param("coo_chunk_records", chunk)
+                _, X = h.ascii_read(p, nprocs, rank, index=index, dtype=dtype, share=s)
+                assert X.nrows == ref.nrows
+                check(X, ref, dtype)
+    h.set_param("coo_chunk_records", 1 << 21)
+
+
+def test_ascii_header_forms_and_errors(h, orc, tmp_path):
+    """"nr nt" headers, '%' lines and a last line without its newline read
+    as the oracle reads them; lines of unequal length and rows out of order
+    return NSP_ERR_INVALID and leave the handle usable."""
+    import nsp_amd
+    from nsp_amd import NspError
+    from nsp_amd._lib import NSP_ERR_INVALID
+    from tests.helpers import write_ascii_coo
+    p = str(tmp_path / "h.txt")
+    write_ascii_coo(p, records=[(1, 3, 1.5), (1, 1, 2.0), (1, 2, -0.5), (3, 3, 4.0)], nr=3, nc=3, header_fields=2,
+                    comments=["%%MatrixMarket matrix coordinate real general", "%"], value_fmt="{:g}",
+                    final_newline=False)
+    _, _, _, _, ref = orc.read_ascii_coo(p)
+    s, X = h.ascii_read(p)
+    assert (s.nr, s.nc, s.nt) == (3, 3, 4)
+    check(X, ref, "f64")
+    bad = str(tmp_path / "unequal.txt")
+    with open(bad, "w") as f:
+        f.write("2 2 2\n 1 1 1.0\n 2 2 10.25\n 1 1 1.0\n")
+    print("  unequal lines", flush=True)
+    with pytest.raises(NspError) as e:
+        h.ascii_read(bad)
+    assert e.value.status == NSP_ERR_INVALID
+    order = str(tmp_path / "order.txt")
+    write_ascii_coo(order, records=[(2, 1, 1.0), (1, 1, 1.0), (2, 2, 1.0)], nr=2, nc=2)
+    s = nsp_amd.ascii_share(order)
+    s.rec_begin, s.rec_end, s.first_row, s.nrows = 0, 3, 0, 2   # the whole file as one share
+    print("  rows out of order", flush=True)
+    with pytest.raises(NspError) as e:
+        h.ascii_read(order, share=s)
+    assert e.value.status == NSP_ERR_INVALID
+    good = ascii_case(tmp_path, "mixed", "{:.17g}")
+    _, X = h.ascii_read(good, 2, 1)
+    check(X, orc.read_ascii_coo(good, 2, 1)[4], "f64")

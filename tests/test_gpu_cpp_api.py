@@ -103,6 +103,28 @@ def test_load_on_device_equals_host_reader(orc, tmp_path, case):
             assert open(host_out, "rb").read() == open(dev_out, "rb").read()
 
 
+def test_ascii_load_through_cpp_api(orc, tmp_path):
+    """load_MatrixMPI with `ascii` set (main.cu's ascii argument,
+    distribute.cu:1084-1086) on an ASCII file of a random matrix with empty
+    rows: every rank's block, written back by the driver, equals the
+    oracle's ASCII reader bit for bit (values through the reference's
+    fast_atof steps)."""
+    from tests.helpers import write_ascii_coo
+    X = gen.random_csr(400, 400, 0.02, seed=8, empty_rows=0.3)
+    path = str(tmp_path / "a.txt")
+    write_ascii_coo(path, X, value_fmt="{:.15e}")
+    for nprocs in (1, 3):
+        for rank in range(nprocs):
+            out = str(tmp_path / f"a{rank}.bin")
+            meta = drive("ascii", path, nprocs, rank, out)
+            nr, nc, nt, shift, ref = orc.read_ascii_coo(path, nprocs, rank)
+            assert (meta["nr"], meta["nc"], meta["nt"], meta["row_shift"]) == (nr, nc, nt, shift)
+            _, _, _, _, got = orc.read_binary_coo(out)
+            np.testing.assert_array_equal(got.rpt, ref.rpt)
+            np.testing.assert_array_equal(got.col, ref.col)
+            np.testing.assert_array_equal(got.val, ref.val)
+
+
 def test_galerkin_driver(orc, tmp_path):
     """galerkin_product through the C++ API: P^T (A P) for the stencil and
     the aggregation prolongator, written back and compared with two oracle

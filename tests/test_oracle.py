@@ -380,6 +380,88 @@ def test_reader_sorts_unsorted_rows(orc, tmp_path):
     assert list(X.col) == [0, 2, 1] and list(X.val) == [2.0, 1.0, 3.0]
 
 
+# ---------------------------------------------------------------- ASCII reader
+def _with_diagonal(A):
+    from nsp_amd.gen import _from_rows_cols
+    n = A.nrows
+    rows = np.concatenate([np.repeat(np.arange(n), np.diff(A.rpt)), np.arange(n)])
+    cols = np.concatenate([A.col, np.arange(n)])
+    key, idx = np.unique(rows * A.ncols + cols, return_index=True)
+    vals = np.concatenate([A.val, np.full(n, 9.0)])[idx]
+    return _from_rows_cols(n, A.ncols, key // A.ncols, key % A.ncols, vals)
+
+
+@pytest.mark.parametrize("nprocs", [1, 2, 3, 4])
+def test_ascii_coo_matches_binary_reader(orc, tmp_path, nprocs):
+    """Values that the reference's parser reads exactly (half-integers: every
+    step, value*10 + d and d/10, is exact): the ASCII block of every rank
+    equals the binary reader's block of the same matrix bit for bit (same
+    split, shift, row order)."""
+    from tests.helpers import write_ascii_coo
+    A = _with_diagonal(gen.random_csr(70, 70, 0.1, seed=4))
+    A = gen.HostCSR(A.rpt, A.col, np.round(A.val * 200) / 2, A.nrows, A.ncols)
+    pa, pb = str(tmp_path / "a.txt"), str(tmp_path / "a.bin")
+    write_ascii_coo(pa, A, value_fmt="{:.1f}")
+    orc.write_binary_coo(pb, A)
+    for r in range(nprocs):
+        ta = orc.read_ascii_coo(pa, nprocs, r)
+        tb = orc.read_binary_coo(pb, nprocs, r)
+        assert ta[:4] == tb[:4]
+        np.testing.assert_array_equal(ta[4].rpt, tb[4].rpt)
+        np.testing.assert_array_equal(ta[4].col, tb[4].col)
+        np.testing.assert_array_equal(ta[4].val, tb[4].val)
+
+
+def test_ascii_header_forms_comments_and_last_line(orc, tmp_path):
+    """'%' lines before the header are skipped, "nr nt" means nc = nr, a file
+    whose last line has no newline still holds nt records, and rows out of
+    column order are sorted stably (check_and_fix_order)."""
+    from tests.helpers import write_ascii_coo
+    recs = [(1, 3, 1.5), (1, 1, 2.0), (1, 2, -0.5), (3, 3, 4.0)]
+    p = str(tmp_path / "h.txt")
+    write_ascii_coo(p, records=recs, nr=3, nc=3, header_fields=2, comments=["%%MatrixMarket x", "% note"],
+                    value_fmt="{:g}", final_newline=False)
+    nr, nc, nt, shift, X = orc.read_ascii_coo(p)
+    assert (nr, nc, nt, shift) == (3, 3, 4, 0)
+    assert list(X.rpt) == [0, 3, 3, 4]
+    assert list(X.col) == [0, 1, 2, 2] and list(X.val) == [2.0, -0.5, 1.5, 4.0]
+
+
+def test_ascii_unequal_lines_rejected(orc, tmp_path):
+    """The reference reads record q at q*row_size (READ_MATRIX_MPI.cpp:482,
+    556); a file with a longer line is read misaligned there and rejected
+    here."""
+    p = str(tmp_path / "u.txt")
+    with open(p, "w") as f:
+        f.write("2 2 2\n 1 1 1.0\n 2 2 10.25\n")
+    with pytest.raises(IOError):
+        orc.read_ascii_coo(p)
+
+
+@pytest.mark.parametrize("text", ["0", "7", "-12", "+33", "9007199254740991", "0.5", "-3.5", "2.5e1", "2.5E-1",
+                                  "3e2", "-4.5e+3", "1e0", "  6.5", "\t-8", "1.0e22", "5e-1", "12.50"])
+def test_fast_atof_exact_inputs(orc, text):
+    """Inputs on which every step of the reference's parser is exact
+    (integer digits, a fraction digit 5 in the first place or 0 after it,
+    powers of ten up to 1e22, which are exact doubles) give the correctly
+    rounded value, as strtod does."""
+    assert orc.fast_atof(text) == float(text)
+
+
+def test_fast_atof_within_a_few_ulps_of_strtod(orc):
+    """On general inputs the parser's separately rounded steps stay within
+    a few units in the last place of the correctly rounded value (a wrong
+    digit weight, exponent sign or scale would miss by orders of
+    magnitude)."""
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-1, 1, 3000) * 10.0 ** rng.integers(-20, 21, 3000)
+    for v in x:
+        for text in (f"{v:.17g}", f"{v:.6e}", f"{v:.9f}"):
+            ref = float(text)
+            got = orc.fast_atof(text)
+            assert abs(got - ref) <= 8 * np.spacing(abs(ref)) + 1e-300, (text, got, ref)
+
+
 # ---------------------------------------------------------------- planted mistakes
 def test_planted_dropped_term_is_caught(orc):
     A = gen.random_csr(30, 30, 0.2, seed=31, integer_values=True)
