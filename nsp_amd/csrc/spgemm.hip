@@ -736,7 +736,9 @@ __global__ __launch_bounds__(256) void k_sym_narrow(int bin_lo, int bin_hi, int6
                                                     int32_t *__restrict__ row_nz) {
   constexpr int RPB = 256 / GS;
   constexpr int WPL = kNarrowWords / GS;  // bitmap words per lane in the count
-  __shared__ __align__(16) uint32_t bits[RPB][kNarrowWords];
+  // rows padded to 640 bytes, so that the groups of a wavefront reading
+  // their bitmaps at the same offsets start half the LDS banks apart
+  __shared__ __align__(16) uint32_t bits[RPB][kNarrowWords + 32];
   __shared__ EntryScratch<GS, float> ent[RPB];
   int64_t count, off;
   bin_range(info->count_sym, bin_lo, bin_hi, direct_rows, count, off);
@@ -784,15 +786,17 @@ __global__ __launch_bounds__(256) void k_sym_narrow(int bin_lo, int bin_hi, int6
       wave_lds_sync();
       int cnt = 0;
       if constexpr (WPL >= 4) {
-        const uint4 *B4 = reinterpret_cast<const uint4 *>(Bm + glane * WPL);
+        // 16-byte words of the bitmap taken lane by lane (consecutive lanes on
+        // consecutive banks; the count needs no order)
+        const uint4 *B4 = reinterpret_cast<const uint4 *>(Bm);
 #pragma unroll
         for (int q = 0; q < WPL / 4; q++) {
-          const uint4 x = B4[q];
+          const uint4 x = B4[q * GS + glane];
           cnt += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
         }
       } else {
 #pragma unroll
-        for (int q = 0; q < WPL; q++) cnt += __popc(Bm[glane * WPL + q]);
+        for (int q = 0; q < WPL; q++) cnt += __popc(Bm[q * GS + glane]);
       }
       cnt = group_sum<GS>(cnt);
       if (glane == 0) row_nz[r] = cnt;
@@ -816,8 +820,12 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
                                                     V *__restrict__ cval) {
   constexpr int RPB = 256 / GS;
   constexpr int WPL = kNarrowWords / GS;
-  // per bitmap word: .x its bits, .y the set bits of the row's words before it
-  __shared__ __align__(16) uint2 wb[RPB][kNarrowWords];
+  // per bitmap word: .x its bits, .y the set bits of the row's words before
+  // it.  Word j of a row sits at wpos(j) = (j % WPL) * GS + j / WPL: the
+  // running count gives lane g words [g*WPL, (g+1)*WPL), and at each step
+  // the group's lanes read consecutive entries; rows are padded to 1152
+  // bytes, so that the groups of a wavefront start half the banks apart.
+  __shared__ __align__(16) uint2 wb[RPB][kNarrowWords + 16];
   __shared__ __align__(16) AccN vals[RPB][NS + GS];  // + one dummy slot per lane
   __shared__ EntryScratch<GS, V> ent[RPB];
   int64_t count, off;
@@ -825,6 +833,7 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
   const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
   uint2 *Wb = wb[slot];
   AccN *Vs = vals[slot];
+  auto wpos = [](uint32_t j) -> uint32_t { return (j % WPL) * GS + j / WPL; };
   // adds a window's products into the value slots of their positions; with
   // `first` (the product that set its column's bit), that product also
   // writes the column into C at its position, so each column is written
@@ -838,7 +847,7 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
     for (int u = 0; u < R; u++) {
       if (!live[u]) continue;
       const uint32_t k = (uint32_t)key[u] & (kNarrowCols - 1);
-      const uint2 e = Wb[k >> 5];
+      const uint2 e = Wb[wpos(k >> 5)];
       const uint32_t rank = e.y + (uint32_t)__popc(e.x & ((1u << (k & 31)) - 1u));
       const uint32_t idx = ok[u] ? rank : (uint32_t)(NS + glane);
       __hip_atomic_fetch_add(&Vs[idx], (AccN)prod[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -895,7 +904,7 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
               // lanes past the end OR nothing into a word of their own
               const uint32_t k = (uint32_t)key[u] & (kNarrowCols - 1);
               const uint32_t bit = ok[u] ? (1u << (k & 31)) : 0u;
-              const uint32_t old = atomicOr(&Wb[ok[u] ? (k >> 5) : (uint32_t)glane].x, bit);
+              const uint32_t old = atomicOr(&Wb[ok[u] ? wpos(k >> 5) : (uint32_t)glane].x, bit);
               sk[u] = key[u];
               sp[u] = prod[u];
               so[u] = ok[u];
@@ -912,13 +921,13 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
         int c = 0;
 #pragma unroll
         for (int w = 0; w < WPL; w++) {
-          cw[w] = (uint32_t)__popc(Wb[glane * WPL + w].x);
+          cw[w] = (uint32_t)__popc(Wb[w * GS + glane].x);
           c += (int)cw[w];
         }
         uint32_t run = (uint32_t)(group_incl_scan<GS>(c, glane) - c);
 #pragma unroll
         for (int w = 0; w < WPL; w++) {
-          Wb[glane * WPL + w].y = run;
+          Wb[w * GS + glane].y = run;
           run += cw[w];
         }
       }
@@ -936,7 +945,7 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
         // the columns from the bits (positions from the running counts)
 #pragma unroll
         for (int w = 0; w < WPL; w++) {
-          const uint2 e = Wb[glane * WPL + w];
+          const uint2 e = Wb[w * GS + glane];
           uint32_t x = e.x, rk = e.y;
           while (x) {
             const int b = __ffs(x) - 1;
