@@ -216,7 +216,7 @@ def fwd_bwd_leg(dtype: str, m: int = 64, steps: int = 50, warmup: int = 5):
 
 
 def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int = 2, params=None,
-                loop: bool = True):
+                loop: bool = True, rounds: int = 5):
     """`count` independent FP32 products C_i = A_i*A_i of uneven sizes: one
     hipGraph launch over the concatenated batch vs the eager batched call
     (same kernels, two host synchronisations) and a per-matrix loop of the
@@ -239,9 +239,14 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
     _, nnz = h.batched_symbolic(dX, d_off, dX, d_off)
     nprod = h.stats()["nprod"]
     g = h.graph(dX, dX, capacity=nnz, a_off=d_off, b_off=d_off)
-    t_graph = _timed(g.launch, steps, warmup)
+    # graph and eager calls timed in alternating rounds (medians), so that
+    # their difference is not drowned by the box's drift between two runs
+    tg, te = [], []
+    for _ in range(rounds):
+        tg.append(_timed(g.launch, steps, warmup))
+        te.append(_timed(lambda: h.batched_spgemm(dX, d_off, dX, d_off), steps, warmup))
+    t_graph, t_eager = float(np.median(tg)), float(np.median(te))
     Cg = g.result()
-    t_eager = _timed(lambda: h.batched_spgemm(dX, d_off, dX, d_off), steps, warmup)
     # phase times of the eager call (HIP events on the handle's stream)
     h.set_param("timing", 1)
     ph = {"t_bin_ms": [], "t_sym_ms": [], "t_scan_ms": [], "t_num_ms": []}

@@ -1867,14 +1867,23 @@ template <typename RowT, typename V>
 static int graph_record(Handle *g, const nsp_csr_t *A, const nsp_csr_t *B, const Batch &bt, nsp_csr_t *C) {
   const int64_t MA = A->rows;
   NSP_TRY(run_rowinfo<RowT>(g, A, B, bt, C->nnz));
+  // grid sizes: the rows a bin may hold (every row), except for the
+  // block-per-row bins, whose grids stop at the blocks a CU holds at once
+  // (their LDS: symbolic 8 and 2, numeric 2 and 1 per CU), so that a bin
+  // left empty by the pattern costs a short launch
+  auto rows_hint = [&](int b, bool num) -> int64_t {
+    const int64_t all = std::max<int64_t>(MA, 1);
+    const int64_t per_cu = b == 6 ? (num ? 2 : 8) : b == 7 ? (num ? 1 : 2) : 0;
+    return per_cu ? std::min<int64_t>(all, per_cu * g->num_cu) : all;
+  };
   launch_scatter(g, MA, g->d_info->count_sym, g->d_info->cursor_sym, g->perm_sym);
   for (int b = 0; b < kBins; b++)
-    launch_sym_bin(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, g->ab_len, B->col, B->nnz, B->cols);
+    launch_sym_bin(g, b, b, -1, rows_hint(b, false), g->a_rpt64, g->ab_b0, g->ab_len, B->col, B->nnz, B->cols);
   hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)(MA / kScanTile + 1)), dim3(kScanBS), 0, g->stream, MA, g->row_nz,
                      (RowT *)C->row_ptr, g->c_rpt64, g->cls, g->tile_status, g->d_info);
   launch_scatter(g, MA, g->d_info->count_num, g->d_info->cursor_num, g->perm_num);
   for (int b = 0; b < kBins; b++)
-    launch_num_bin<V>(g, b, b, -1, std::max<int64_t>(MA, 1), g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len,
+    launch_num_bin<V>(g, b, b, -1, rows_hint(b, true), g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len,
                       B->col, (const V *)B->val, C->col, (V *)C->val, B->cols, B->nnz);
   NSP_HIP(g, hipGetLastError());
   return NSP_OK;
