@@ -119,6 +119,33 @@ def test_heavy_rows_in_graph(nsp, orc):
         torch.cuda.synchronize()
 
 
+def test_block_bins_in_graph_more_rows_than_grid(nsp, orc):
+    """The graph launches the block-per-row bins on the grid a CU holds at
+    once (2048 symbolic blocks for bin 6, 512 for bin 7 on 256 CUs; 512 and
+    256 numeric): here those bins hold more rows than that, so each block
+    walks several rows, and the product must still match the oracle on
+    every replay."""
+    print("case: block bins in graph", flush=True)
+    import torch
+    rng = np.random.default_rng(31)
+    lens = np.concatenate([rng.integers(16, 25, 2600), rng.integers(30, 46, 1200)])
+    A = rows_matrix(lens, 4000, seed=32)
+    B = rows_matrix(np.full(4000, 80), 400000, seed=33)
+    ref = orc.spgemm(A, B)
+    h = nsp.Handle()
+    dA, dB = nsp.DeviceCSR.from_host(A), nsp.DeviceCSR.from_host(B)
+    h.spgemm(dA, dB)
+    st = h.stats()
+    assert st["sym_bin"][6] > 2048 and st["sym_bin"][7] > 512, st["sym_bin"]
+    assert st["num_bin"][6] > 512 and st["num_bin"][7] > 256, st["num_bin"]
+    g = h.graph(dA, dB, capacity=int(ref[0][-1]))
+    for _ in range(2):
+        g.launch()
+        check_against_oracle(A, B, g.result(), "f64", orc, ref=ref)
+        (torch.randn(1000, device="cuda") * 2).sum().item()
+        torch.cuda.synchronize()
+
+
 def check_sampled_rows(nsp, orc, A, B, C, rows, dtype):
     """Rows `rows` of C (device) against the oracle's product of those rows."""
     import torch
