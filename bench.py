@@ -611,27 +611,39 @@ def main():
     st = h.stats()
     nprod = st["nprod"]
 
+    # one step = one whole product C = A*B into C's buffers: the one-call
+    # entry point (nsp_spgemm_into: one numeric pass into per-row slots, scan,
+    # copy into C, when a row sample allows; else symbolic + numeric)
+    Cbuf = nsp_amd.DeviceCSR(rpt, C.col, C.val, C.nrows, C.ncols)
+
     def step():
+        Cs = h.spgemm_into(dA, dB, out=Cbuf)
+        assert Cs.col.numel() == nnz
+        return h.stats()
+
+    def step_two_phase():
         r, z = h.symbolic(dA, dB)
         assert z == nnz
         h.numeric(dA, dB, r, z, out=nsp_amd.DeviceCSR(r, C.col, C.val, C.nrows, C.ncols))
         return h.stats()
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t_num, t_sym, t_scan, t_bin = [], [], [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s = step()
-        t_num.append(s["t_num_ms"])
-        t_sym.append(s["t_sym_ms"])
-        t_scan.append(s["t_scan_ms"])
-        t_bin.append(s["t_bin_ms"])
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms = dt * 1e3 / args.steps
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        ph = {k: [] for k in ("t_num_ms", "t_sym_ms", "t_scan_ms", "t_bin_ms", "t_compact_ms")}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s = fn()
+            for k in ph:
+                ph[k].append(s[k])
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps, {k: float(np.mean(v)) for k, v in ph.items()}, s
+
+    ms, ph, st_step = timed(step)
+    ms2, ph2, _ = timed(step_two_phase)
     gflops = 2.0 * nprod / (ms / 1e3) / 1e9
+    t_num, t_sym, t_scan, t_bin = [ph["t_num_ms"]], [ph["t_sym_ms"]], [ph["t_scan_ms"]], [ph["t_bin_ms"]]
 
     sym_b, scan_b, num_b = algorithmic_bytes(A, B, nnz, 8)
     traffic = measured_traffic(args.workload)
@@ -651,10 +663,17 @@ def main():
                      "algorithmic_bytes": num_b, "kernel_ms": round(t_num_avg, 4)},
         "phases_ms": {"binning": round(float(np.mean(t_bin)), 4),
                       "symbolic": round(float(np.mean(t_sym)), 4),
-                      "scan": round(float(np.mean(t_scan)), 4), "numeric": round(t_num_avg, 4)},
+                      "scan": round(float(np.mean(t_scan)), 4), "numeric": round(t_num_avg, 4),
+                      "compaction": round(ph["t_compact_ms"], 4)},
+        "one_call": {"fused": st_step["fused"],
+                     "note": "numeric pass into per-row slots (no symbolic pass), scan, copy into C"
+                     if st_step["fused"] else "symbolic + numeric (the row sample ruled the slots out)"},
+        "two_phase": {"ms_per_step": round(ms2, 4), "GFLOPs": round(2.0 * nprod / (ms2 / 1e3) / 1e9, 2),
+                      "phases_ms": {"binning": round(ph2["t_bin_ms"], 4), "symbolic": round(ph2["t_sym_ms"], 4),
+                                    "scan": round(ph2["t_scan_ms"], 4), "numeric": round(ph2["t_num_ms"], 4)}},
         "two_pass_bytes": sym_b + scan_b + num_b,
         "two_pass_GBs_per_step": round((sym_b + scan_b + num_b) / (ms / 1e3) / 1e9, 1),
-        "host_syncs_per_step": st["host_syncs"],
+        "host_syncs_per_step": st_step["host_syncs"],
         "params": params,
         "bins": {"sym": st["sym_bin"], "num": st["num_bin"], "sym_launches": st["sym_launches"],
                  "num_launches": st["num_launches"], "direct_mode": st["direct_mode"]},

@@ -78,6 +78,11 @@ typedef struct {
    * conversion kernel alone) or nsp_coo_read (first chunk's conversion to
    * the last one's end, copies included), when "timing" is 1 */
   float t_coo_ms;
+  /* 1 when the last one-call product (nsp_spgemm, nsp_spgemm_into) ran its
+   * numeric pass into per-row slots (no symbolic pass); t_num_ms is then
+   * that pass, t_sym_ms 0, and t_compact_ms the copy of the slots into C */
+  int32_t fused;
+  float t_compact_ms;
 } nsp_stats_t;
 
 /* ---- handle ------------------------------------------------------------ */
@@ -105,6 +110,13 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *   "narrow_mode"  1 (default): when B has at most 4096 columns, rows of up
  *                  to 1024 products / 512 entries keep their columns in an
  *                  LDS bitmap (no hash table, no sort); 0: hash tables
+ *   "fused_mode"   1 (default): one-call products (nsp_spgemm, nsp_spgemm_into)
+ *                  skip the symbolic pass when a sample of 1024 rows fits a
+ *                  wave-level numeric bin (at most 512 columns): the numeric
+ *                  pass writes each row into a slot of that bin's size and
+ *                  counts it, a scan gives C's row pointer, and a copy packs
+ *                  the slots into C (a row that outgrows its slot sends the
+ *                  product back to two passes); 0: always two passes
  *   "sym_table_factor"   symbolic hash slots per intermediate product in the
  *                  bins of up to 256 products (1, 2; default 2: rows whose
  *                  products are mostly distinct keep the table half empty)
@@ -154,6 +166,19 @@ typedef void (*nsp_free_fn)(void *ptr, void *user);
 int nsp_spgemm(nsp_handle_t h, nsp_index_t it, nsp_value_t vt,
                const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C,
                nsp_alloc_fn alloc, nsp_free_fn dealloc, void *user);
+
+/* One-call product into caller buffers: C->row_ptr (A->rows+1 entries) and
+ * C->col / C->val with a capacity of C->nnz entries; *c_nnz gets nnz(C).
+ * Same result as nsp_spgemm (and as nsp_spgemm_symbolic + _numeric), in
+ * one pass over the products when the "fused_mode" sample allows.  C larger
+ * than the capacity returns NSP_ERR_OVERFLOW with *c_nnz set and the row
+ * pointer filled (C->col / C->val then hold an unspecified prefix; call
+ * again with room).
+ * Replaces nsp_spgemm_kernel_hash's computation (nsp/src/nsp.cu:2448-2488)
+ * for callers that own C's buffers. */
+int nsp_spgemm_into(nsp_handle_t h, nsp_index_t it, nsp_value_t vt,
+                    const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C,
+                    int64_t *c_nnz);
 
 /* Bytes of the device array that nsp_spgemm_noalloc uses as its scratch
  * (one copy per device, held until the process ends). */

@@ -113,7 +113,7 @@ class Handle:
                 "host_syncs": st.host_syncs, "sym_launches": st.sym_launches,
                 "num_launches": st.num_launches, "t_bin_ms": st.t_bin_ms,
                 "t_sym_ms": st.t_sym_ms, "t_scan_ms": st.t_scan_ms, "t_num_ms": st.t_num_ms,
-                "t_coo_ms": st.t_coo_ms}
+                "t_coo_ms": st.t_coo_ms, "fused": st.fused, "t_compact_ms": st.t_compact_ms}
 
     # -- local product ---------------------------------------------------
     def symbolic(self, A: DeviceCSR, B: DeviceCSR):
@@ -142,6 +142,38 @@ class Handle:
     def spgemm(self, A: DeviceCSR, B: DeviceCSR) -> DeviceCSR:
         rpt, nnz = self.symbolic(A, B)
         return self.numeric(A, B, rpt, nnz)
+
+    def spgemm_into(self, A: DeviceCSR, B: DeviceCSR, out: DeviceCSR | None = None,
+                    capacity: int | None = None) -> DeviceCSR:
+        """One-call product (nsp_spgemm_into) into `out`'s buffers (row
+        pointer of A.nrows+1 entries, col/val of their length as the
+        capacity), or into new buffers of `capacity` entries; a C larger
+        than the capacity is counted, the buffers are regrown and the call
+        repeated.  Returns C as a DeviceCSR viewing the first nnz(C) entries."""
+        torch = _torch()
+        _same_types(A, B)
+        dev = A.col.device
+        if out is None:
+            cap = int(capacity or 0)
+            out = DeviceCSR(torch.empty(A.nrows + 1, dtype=A.row_ptr.dtype, device=dev),
+                            torch.empty(cap, dtype=torch.int32, device=dev),
+                            torch.empty(cap, dtype=A.val.dtype, device=dev), A.nrows, B.ncols)
+        for _ in range(2):
+            c = _lib.nsp_csr_t(A.nrows, B.ncols, out.col.numel(), out.row_ptr.data_ptr(),
+                               out.col.data_ptr() if out.col.numel() else None,
+                               out.val.data_ptr() if out.val.numel() else None)
+            nnz = C.c_int64()
+            a, b = A.c_struct(), B.c_struct()
+            st = self.L.nsp_spgemm_into(self.h, A.index_type, A.value_type, C.byref(a), C.byref(b),
+                                        C.byref(c), C.byref(nnz))
+            if st == _lib.NSP_ERR_OVERFLOW and nnz.value > out.col.numel():  # regrow and repeat
+                out = DeviceCSR(out.row_ptr, torch.empty(nnz.value, dtype=torch.int32, device=dev),
+                                torch.empty(nnz.value, dtype=A.val.dtype, device=dev), A.nrows, B.ncols)
+                continue
+            self._check(st)
+            n = int(nnz.value)
+            return DeviceCSR(out.row_ptr, out.col[:n], out.val[:n], A.nrows, B.ncols)
+        raise NspError(_lib.NSP_ERR_OVERFLOW, "C grew between two counts")
 
     def intprod(self, A: DeviceCSR, B: DeviceCSR):
         torch = _torch()

@@ -29,7 +29,8 @@ class nsp_stats_t(C.Structure):
                 ("n_num_bins", C.c_int32), ("direct_mode", C.c_int32), ("host_syncs", C.c_int32),
                 ("sym_launches", C.c_int32), ("num_launches", C.c_int32),
                 ("t_bin_ms", C.c_float), ("t_sym_ms", C.c_float), ("t_scan_ms", C.c_float),
-                ("t_num_ms", C.c_float), ("t_coo_ms", C.c_float)]
+                ("t_num_ms", C.c_float), ("t_coo_ms", C.c_float),
+                ("fused", C.c_int32), ("t_compact_ms", C.c_float)]
 
 
 class nsp_coo_share_t(C.Structure):
@@ -68,6 +69,7 @@ SIGNATURES = [
     ("nsp_spgemm_numeric", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP]),
     ("nsp_spgemm", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _VP, _VP, _VP]),
     ("nsp_spgemm_noalloc", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _P64]),
+    ("nsp_spgemm_into", _I, [_H, _I, _I, _CSRP, _CSRP, _CSRP, _P64]),
     ("nsp_csr_free", _I, [_CSRP]),
     ("nsp_intprod", _I, [_H, _I, _CSRP, _CSRP, _VP, _P64]),
     ("nsp_check_csr", _I, [_H, _I, _CSRP, _P64]),

@@ -115,6 +115,7 @@ int nsp_destroy(nsp_handle_t hs) {
   if (h->segw.base) (void)hipFree(h->segw.base);
   if (h->coo.base) (void)hipFree(h->coo.base);
   if (h->rsort.base) (void)hipFree(h->rsort.base);
+  if (h->spec.base) (void)hipFree(h->spec.base);
   if (h->pin) (void)hipHostFree(h->pin);
   for (hipEvent_t e : h->pin_ev)
     if (e) (void)hipEventDestroy(e);
@@ -188,6 +189,9 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
   } else if (!std::strcmp(name, "narrow_mode")) {
     if (value != 0 && value != 1) return nsp::set_error(h, NSP_ERR_INVALID, "narrow_mode must be 0 or 1");
     h->narrow_mode = (int)value;
+  } else if (!std::strcmp(name, "fused_mode")) {
+    if (value < 0 || value > 1) return nsp::set_error(h, NSP_ERR_INVALID, "fused_mode must be 0 or 1");
+    h->fused_mode = (int)value;
   } else if (!std::strcmp(name, "sym_table_factor")) {
     if (value != 1 && value != 2) return nsp::set_error(h, NSP_ERR_INVALID, "sym_table_factor must be 1 or 2");
     h->sym_table_factor = (int)value;
@@ -244,6 +248,34 @@ int nsp_spgemm_noalloc(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const ns
   return nsp::spgemm_noalloc(&hs->h, it, vt, A, B, C, nnz);
 }
 
+int nsp_spgemm_into(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                    nsp_csr_t *C, int64_t *c_nnz) {
+  if (!hs || !C || !c_nnz) return NSP_ERR_INVALID;
+  Handle *h = &hs->h;
+  if (!C->row_ptr) return nsp::set_error(h, NSP_ERR_INVALID, "C.row_ptr is NULL");
+  if (C->nnz < 0) return nsp::set_error(h, NSP_ERR_INVALID, "C.nnz (the capacity) is negative");
+  const int64_t cap = C->nnz;
+  if (cap > 0 && (!C->col || !C->val)) return nsp::set_error(h, NSP_ERR_INVALID, "C.col/C.val is NULL");
+  int64_t nnz = 0;
+  bool filled = false;
+  nsp_csr_t into = *C;
+  int s = nsp::spgemm_fused_count(h, it, vt, A, B, C->row_ptr, &nnz, &into, &filled);
+  if (s != NSP_OK) return s;
+  *c_nnz = nnz;
+  if (nnz > cap) {
+    h->fused_plan = false;
+    h->have_plan = false;
+    return nsp::set_error(h, NSP_ERR_OVERFLOW, "C needs " + std::to_string(nnz) + " entries, more than its capacity " +
+                                                   std::to_string(cap) + " (C.row_ptr is filled)");
+  }
+  if (filled) return NSP_OK;
+  nsp_csr_t c = *C;
+  c.rows = A->rows;
+  c.cols = B->cols;
+  c.nnz = nnz;
+  return nsp::spgemm_fused_fill(h, it, vt, A, B, &c);
+}
+
 int nsp_spgemm(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
                nsp_csr_t *C, nsp_alloc_fn alloc, nsp_free_fn dealloc, void *user) {
   if (!hs || !C) return NSP_ERR_INVALID;
@@ -256,7 +288,7 @@ int nsp_spgemm(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t 
   void *rpt = alloc((size_t)(A->rows + 1) * isz, user);
   if (!rpt) return nsp::set_error(h, NSP_ERR_OOM, "C.row_ptr allocation failed");
   int64_t nnz = 0;
-  int s = nsp::spgemm_symbolic(h, it, vt, A, B, rpt, &nnz);
+  int s = nsp::spgemm_fused_count(h, it, vt, A, B, rpt, &nnz);
   if (s != NSP_OK) {
     dealloc(rpt, user);
     return s;
@@ -275,7 +307,7 @@ int nsp_spgemm(nsp_handle_t hs, nsp_index_t it, nsp_value_t vt, const nsp_csr_t 
   C->row_ptr = rpt;
   C->col = (int32_t *)col;
   C->val = val;
-  s = nsp::spgemm_numeric(h, it, vt, A, B, C);
+  s = nsp::spgemm_fused_fill(h, it, vt, A, B, C);
   if (s == NSP_OK && hipStreamSynchronize(h->stream) != hipSuccess)
     s = nsp::set_error(h, NSP_ERR_HIP, hipGetErrorString(hipGetLastError()));
   if (s != NSP_OK) {

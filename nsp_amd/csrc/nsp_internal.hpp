@@ -35,7 +35,8 @@ struct Info {
   uint32_t error;
   int64_t cap;  // capacity of C's column/value buffers (numeric rows beyond it are skipped)
   int64_t nlong;  // rows binned by the long-row pass
-  int64_t pad[2];
+  int64_t smp_max;    // one-call product: most distinct columns of a sampled row
+  int64_t spec_over;  // one-call product: a row had more columns than its slot
 };
 
 // A batch of independent products stored as concatenated rows: matrix b owns
@@ -76,6 +77,7 @@ struct Handle {
   int num_table_factor = 2;
   int sym_table_factor = 2;  // symbolic slots per product bound in bins 0..3 (1 or 2)
   int narrow_mode = 1;       // 1: column bitmaps instead of hash tables when B has at most 4096 columns
+  int fused_mode = 1;        // one-call products: 1 numeric pass into per-row slots + compaction when a sample allows
   int fp32_accumulate = 0;   // 1: FP32 values summed in FP32 where the kernel offers it (narrow products)
   int64_t coo_chunk = int64_t(1) << 21;  // records per streamed chunk of the COO reader
   bool timing = false;
@@ -90,6 +92,7 @@ struct Handle {
   Workspace segw;  // segment sort: tile lists, merge buffer
   Workspace rsort; // row sort: extents, keys, values of the rows being sorted
   Workspace coo;   // COO conversion: row bitmap, unsorted-row list, staging
+  Workspace spec;  // one-call product: C rows in fixed-size slots before compaction
   void *pin = nullptr;  // pinned host staging of the COO reader
   size_t pin_bytes = 0;
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -104,6 +107,12 @@ struct Handle {
 
   // state carried from symbolic to numeric
   bool have_plan = false;
+  // one-call product: C's rows wait in `spec` (slots of fused_ns entries)
+  // for the compaction into the caller's buffers
+  bool fused_plan = false;
+  bool timing_fused = false;
+  int64_t fused_ns = 0;
+  const void *fused_key[2] = {nullptr, nullptr};
   int64_t plan_rows = -1, plan_nnz = -1;
   const void *plan_a = nullptr, *plan_b = nullptr;
   int plan_sub = 16;

@@ -588,7 +588,7 @@ using Acc = double;
 // Numeric, GS lanes per row, SH-slot LDS table (keys + values) per row, then
 // compaction of the occupied slots, a register bitonic sort of the row's
 // columns (GS lanes x E keys) and a lookup of each sorted column's value.
-template <typename V, int GS, int NS, int SHF, int R, bool O32>
+template <typename V, int GS, int NS, int SHF, int R, bool O32, bool SPEC = false>
 __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64_t direct_rows,
                                                    const Info *__restrict__ info,
                                                    const int32_t *__restrict__ perm,
@@ -600,7 +600,8 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
                                                    const V *__restrict__ bval,
                                                    const int64_t *__restrict__ crpt,
                                                    int32_t *__restrict__ ccol,
-                                                   V *__restrict__ cval, int pack) {
+                                                   V *__restrict__ cval, int pack,
+                                                   int32_t *__restrict__ spec_nz) {
   constexpr int RPB = 256 / GS;
   constexpr int SH = NS * SHF;    // table slots (NS = largest row nnz of the bin)
   constexpr int LOG = Log2<SH>::value;
@@ -645,9 +646,11 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
         an0 = arpt[rn];
         an1 = arpt[rn + 1];
       }
-      const int64_t c0 = crpt[r];
-      const int nz = (int)(crpt[r + 1] - c0);
-      if (c0 + nz > info->cap) {  // C buffers too small: skip the row, flag it
+      // SPEC (one-call product, C's row pointer not known yet): the row
+      // goes to slot r of NS entries of ccol/cval, its length to spec_nz
+      const int64_t c0 = SPEC ? (int64_t)r * NS : crpt[r];
+      int nz = SPEC ? 0 : (int)(crpt[r + 1] - c0);
+      if (!SPEC && c0 + nz > info->cap) {  // C buffers too small: skip the row, flag it
         if (glane == 0) atomicOr((unsigned int *)&const_cast<Info *>(info)->error, (unsigned int)NSP_ERR_OVERFLOW);
         if (rn >= 0) pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
         continue;
@@ -672,6 +675,15 @@ __global__ __launch_bounds__(256) void k_num_group(int bin_lo, int bin_hi, int64
 #pragma unroll
       for (int q = 0; q < SPL; q++) c += (K[q * GS + glane] != kEmpty);
       int pos = group_incl_scan<GS>(c, glane) - c;
+      if constexpr (SPEC) {
+        nz = group_last<GS>(pos + c);
+        if (glane == 0) spec_nz[r] = nz > NS ? 0 : nz;
+        if (nz > NS) {  // more columns than the slot and the sort hold: the caller redoes the product
+          if (glane == 0) atomicOr((unsigned int *)&const_cast<Info *>(info)->spec_over, 1u);
+          wave_lds_sync();
+          continue;
+        }
+      }
 #pragma unroll
       for (int q = 0; q < SPL; q++) {
         const int32_t k = K[q * GS + glane];
@@ -1281,6 +1293,69 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
 }
 
 // ---------------------------------------------------------------------------
+// One-call product in one hash pass.  k_sym_sample counts the distinct
+// columns of up to kSpecSample evenly spaced rows (one wavefront per row,
+// a 2048-slot LDS table; rows past 1024 products are left out, as such a
+// product does not take this path) and keeps the largest count.  When every
+// row of C is then expected to fit a numeric bin's NS entries, k_num_group
+// runs once over all rows with `spec` = NS, writing row r to slot r of a
+// buffer of NS entries per row and its length to row_nz; k_scan turns the
+// lengths into C's row pointer, and k_spec_compact copies the slots into C.
+// A row with more than NS columns is flagged (Info::spec_over) and the
+// product is redone in two passes.
+constexpr int64_t kSpecSample = 1024;
+
+template <int R>
+__global__ __launch_bounds__(256) void k_sym_sample(int64_t MA, int64_t nsample, const int64_t *__restrict__ arpt,
+                                                    const int64_t *__restrict__ intprod,
+                                                    const int64_t *__restrict__ ab_b0,
+                                                    const int32_t *__restrict__ ab_len,
+                                                    const int32_t *__restrict__ bcol, Info *info) {
+  constexpr int SH = 2048;
+  constexpr int LOG = Log2<SH>::value;
+  __shared__ int32_t keys[4][SH + 64];
+  __shared__ EntryScratch<64, float> ent[4];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int32_t *K = keys[w];
+  const int64_t stride = MA / nsample;
+  for (int64_t t = blockIdx.x * 4 + w; t < nsample; t += (int64_t)gridDim.x * 4) {
+    const int64_t r = t * stride;
+    if (intprod[r] > SH / 2) continue;
+    clear_lds<64, SH + 64>(K, lane, kEmpty);
+    wave_lds_sync();
+    int cnt = 0;
+    row_products<64, R, false, float>(
+        lane, arpt[r], arpt[r + 1], 0, 64, ab_b0, ab_len, nullptr, bcol, nullptr, &ent[w],
+        [&](const int32_t(&key)[R], const float(&)[R], const bool(&ok)[R], auto &&...) {
+          cnt += hash_insert_keys<LOG, R>(K, key, ok, lane);
+        });
+    cnt = group_sum<64>(cnt);
+    if (lane == 0) atomicMax((unsigned long long *)&info->smp_max, (unsigned long long)cnt);
+    wave_lds_sync();
+  }
+}
+
+// Copies row r's slot (NS entries from r*NS) to its place in C, one
+// wavefront per row, launched with a wavefront for every row (the row's
+// extent is a dependent load before its copy, so rows are not looped).
+template <typename V>
+__global__ __launch_bounds__(256) void k_spec_compact(int64_t MA, int64_t NS, int64_t cap,
+                                                      const int64_t *__restrict__ crpt,
+                                                      const int32_t *__restrict__ scol, const V *__restrict__ sval,
+                                                      int32_t *__restrict__ ccol, V *__restrict__ cval) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; r < MA;
+       r += ((int64_t)gridDim.x * blockDim.x) >> 6) {
+    const int64_t c0 = crpt[r], s0 = r * NS;
+    const int64_t n = (crpt[r + 1] < cap ? crpt[r + 1] : cap) - c0;
+    for (int64_t j = lane; j < n; j += 64) {
+      ccol[c0 + j] = scol[s0 + j];
+      cval[c0 + j] = sval[s0 + j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host-side launch tables
 static int grid_for(const Handle *h, int64_t items, int per_block, int blocks_per_cu) {
   int64_t need = (items + per_block - 1) / per_block;
@@ -1478,7 +1553,8 @@ template <typename V>
 static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t count,
                            const int64_t *arpt, const int64_t *ab_b0, const V *aval,
                            const int32_t *ab_len, const int32_t *bcol, const V *bval,
-                           int32_t *ccol, V *cval, int64_t ncols, int64_t nnzB) {
+                           int32_t *ccol, V *cval, int64_t ncols, int64_t nnzB, int spec = 0,
+                           int32_t *spec_nz = nullptr) {
   if (h->narrow_mode && ncols <= kNarrowCols && bin <= 5) {
     const int rpw = bin <= 2 ? h->num_small_rows_per_wave : h->num_rows_per_wave;
     const bool o32 = offsets32(nnzB);
@@ -1502,15 +1578,26 @@ static void launch_num_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
   const int64_t *crpt = h->c_rpt64;
   const int cap = h->grid_cap;
   const bool o32 = offsets32(nnzB);
+  // `spec` (the one-call product's slot pass) exists for 32-bit B offsets
+  // and table factor 2 only; the caller sets both
 #define NUM_GROUP(GS, NS, SHF)                                                                      \
-  if (o32)                                                                                          \
-    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, num_r(GS, NS), true>), dim3(grid_for(h, count, 256 / GS, cap)), \
-                       dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,    \
-                       bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF));                     \
-  else                                                                                              \
-    hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, num_r(GS, NS), false>), dim3(grid_for(h, count, 256 / GS, cap)), \
-                       dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,    \
-                       bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF))
+  do {                                                                                              \
+    if (spec) {                                                                                     \
+      if constexpr ((SHF) == 2)                                                                     \
+        hipLaunchKernelGGL((k_num_group<V, GS, NS, 2, num_r(GS, NS), true, true>),                  \
+                           dim3(grid_for(h, count, 256 / GS, cap)), dim3(256), 0, st, lo, bin, direct, info, \
+                           perm, arpt, ab_b0, aval, ab_len, bcol, bval, crpt, ccol, cval,          \
+                           pack_ok(ncols, NS * 2), spec_nz);                                        \
+    } else if (o32) {                                                                               \
+      hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, num_r(GS, NS), true>), dim3(grid_for(h, count, 256 / GS, cap)), \
+                         dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,  \
+                         bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF), nullptr);          \
+    } else {                                                                                        \
+      hipLaunchKernelGGL((k_num_group<V, GS, NS, SHF, num_r(GS, NS), false>), dim3(grid_for(h, count, 256 / GS, cap)), \
+                         dim3(256), 0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len,  \
+                         bcol, bval, crpt, ccol, cval, pack_ok(ncols, NS * SHF), nullptr);          \
+    }                                                                                               \
+  } while (0)
 #define NUM_BLOCK(BS, NS, SHF)                                                                      \
   hipLaunchKernelGGL((k_num_block<V, BS, NS, SHF, kNumR>), dim3(grid_for(h, count, 1, cap)), dim3(BS),   \
                      0, st, lo, bin, direct, info, perm, arpt, ab_b0, aval, ab_len, bcol, bval,     \
@@ -1607,8 +1694,18 @@ static void timing_mark(Handle *h, int i) {
 
 int collect_timing(Handle *h) {
   if (!h->timing || !h->timing_pending) return NSP_OK;
-  NSP_HIP(h, hipEventSynchronize(h->events[5]));
   float t = 0;
+  if (h->timing_fused) {  // one-call product: binning + sample, numeric into slots, scan, compaction
+    NSP_HIP(h, hipEventSynchronize(h->events[7]));
+    if (hipEventElapsedTime(&t, h->events[0], h->events[1]) == hipSuccess) h->stats.t_bin_ms = t;
+    h->stats.t_sym_ms = 0;
+    if (hipEventElapsedTime(&t, h->events[4], h->events[5]) == hipSuccess) h->stats.t_num_ms = t;
+    if (hipEventElapsedTime(&t, h->events[5], h->events[3]) == hipSuccess) h->stats.t_scan_ms = t;
+    if (hipEventElapsedTime(&t, h->events[6], h->events[7]) == hipSuccess) h->stats.t_compact_ms = t;
+    h->timing_pending = false;
+    return NSP_OK;
+  }
+  NSP_HIP(h, hipEventSynchronize(h->events[5]));
   if (hipEventElapsedTime(&t, h->events[0], h->events[1]) == hipSuccess) h->stats.t_bin_ms = t;
   if (hipEventElapsedTime(&t, h->events[1], h->events[2]) == hipSuccess) h->stats.t_sym_ms = t;
   if (hipEventElapsedTime(&t, h->events[2], h->events[3]) == hipSuccess) h->stats.t_scan_ms = t;
@@ -1718,13 +1815,16 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
 
 template <typename RowT>
 static int symbolic_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void *c_row_ptr,
-                         int64_t *c_nnz, const Batch &bt = Batch{}) {
+                         int64_t *c_nnz, const Batch &bt = Batch{}, bool info_ready = false) {
   const int64_t MA = A->rows;
-  h->stats = nsp_stats_t{};
-  h->have_plan = false;
-  h->timing_pending = false;
-  NSP_TRY(run_rowinfo<RowT>(h, A, B, bt));
-  NSP_TRY(read_info(h));
+  if (!info_ready) {  // else the row info of this product is in place and read back
+    h->stats = nsp_stats_t{};
+    h->have_plan = false;
+    h->timing_pending = false;
+    h->timing_fused = false;
+    NSP_TRY(run_rowinfo<RowT>(h, A, B, bt));
+    NSP_TRY(read_info(h));
+  }
   const Info &hi = *h->h_info;
   {
     Launch L[kBins];
@@ -1803,6 +1903,7 @@ static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
   h->stats.num_launches = n;
   h->stats.direct_mode = (h->stats.direct_mode & 1) | ((n == 1 && L[0].direct >= 0) ? 2 : 0);
   timing_mark(h, 5);
+  h->timing_fused = false;
   h->timing_pending = h->timing;
   NSP_HIP(h, hipGetLastError());
   return NSP_OK;
@@ -1820,6 +1921,164 @@ int spgemm_numeric(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A
   if (vt == NSP_F64) return numeric_impl<double>(h, A, B, C);
   if (vt == NSP_F32) return numeric_impl<float>(h, A, B, C);
   return set_error(h, NSP_ERR_INVALID, "unknown value type");
+}
+
+// ---------------------------------------------------------------------------
+// One-call product (nsp_spgemm_into, nsp_spgemm): C = A*B with the numeric
+// pass as the only hash pass when a sample of rows says every row of C fits
+// a wave-level numeric bin (see k_sym_sample).  spgemm_fused_count leaves
+// C's row pointer and nnz, with the rows in the handle's slot buffer;
+// spgemm_fused_fill copies them into C.  Products the sample rules out
+// (rows past 1024 products, column-bitmap products, sampled rows of more
+// than 512 columns) and products where a row outgrows its slot take the
+// two-pass path, whose numeric pass spgemm_fused_fill then runs.
+constexpr size_t kSpecMaxBytes = size_t(16) << 30;
+
+// Copies the slots in the handle's buffer (rows of NS entries) into C's
+// col/val (capacity cap; entries past it are not written).
+static void launch_spec_compact(Handle *h, nsp_value_t vt, int64_t MA, int64_t NS, int64_t cap, int32_t *ccol,
+                                void *cval) {
+  if (MA <= 0 || cap <= 0) return;
+  const size_t cbytes = ((size_t)MA * NS * 4 + 255) & ~(size_t)255;
+  const int32_t *scol = (const int32_t *)h->spec.base;
+  const void *sval = (const char *)h->spec.base + cbytes;
+  const dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((MA + 3) / 4, INT32_MAX)));
+  if (vt == NSP_F64)
+    hipLaunchKernelGGL(k_spec_compact<double>, grid, dim3(256), 0, h->stream, MA, NS, cap, h->c_rpt64, scol,
+                       (const double *)sval, ccol, (double *)cval);
+  else
+    hipLaunchKernelGGL(k_spec_compact<float>, grid, dim3(256), 0, h->stream, MA, NS, cap, h->c_rpt64, scol,
+                       (const float *)sval, ccol, (float *)cval);
+}
+
+template <typename RowT, typename V>
+static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, void *c_row_ptr, int64_t *c_nnz,
+                            nsp_csr_t *into, bool *filled) {
+  const int64_t MA = A->rows;
+  h->fused_plan = false;
+  h->timing_fused = false;
+  if (filled) *filled = false;
+  if (h->fused_mode == 0 || MA == 0 || A->nnz == 0 || B->nnz == 0 || !offsets32(B->nnz) ||
+      (h->narrow_mode && B->cols <= kNarrowCols))
+    return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz);
+  h->stats = nsp_stats_t{};
+  h->have_plan = false;
+  h->timing_pending = false;
+  NSP_TRY(run_rowinfo<RowT>(h, A, B));
+  const int64_t ns = std::min<int64_t>(MA, kSpecSample);
+  hipLaunchKernelGGL(k_sym_sample<kSymR>, dim3((unsigned)std::min<int64_t>((ns + 3) / 4, (int64_t)h->num_cu * 4)),
+                     dim3(256), 0, h->stream, MA, ns, h->a_rpt64, h->intprod, h->ab_b0, h->ab_len, B->col, h->d_info);
+  NSP_HIP(h, hipGetLastError());
+  timing_mark(h, 1);
+  NSP_TRY(read_info(h));
+  const Info &hi = *h->h_info;
+  // the smallest wave-level numeric bin whose NS holds the sampled rows (a
+  // row past it is caught and redone in two passes)
+  int bin = -1;
+  if (hi.count_sym[6] + hi.count_sym[7] + hi.count_sym[8] == 0 && hi.smp_max > 0)
+    for (int b = 0; b <= 5 && bin < 0; b++)
+      if (hi.smp_max <= kNumBound[b]) bin = b;
+  const int64_t NS = bin >= 0 ? kNumBound[bin] : 0;
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t cbytes = al((size_t)MA * NS * 4), need = cbytes + al((size_t)MA * NS * sizeof(V));
+  if (bin < 0 || need > kSpecMaxBytes) return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz, Batch{}, true);
+  if (need > h->spec.bytes) {
+    if (h->spec.base) NSP_HIP(h, hipFree(h->spec.base));
+    h->spec.base = nullptr;
+    h->spec.bytes = 0;
+    if (hipMalloc(&h->spec.base, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz, Batch{}, true);
+    }
+    h->spec.bytes = need;
+  }
+  int32_t *scol = (int32_t *)h->spec.base;
+  V *sval = (V *)((char *)h->spec.base + cbytes);
+  timing_mark(h, 4);
+  const int ntf = h->num_table_factor;
+  h->num_table_factor = 2;
+  launch_num_bin<V>(h, bin, bin, MA, MA, h->a_rpt64, h->ab_b0, (const V *)A->val, h->ab_len, B->col,
+                    (const V *)B->val, scol, sval, B->cols, B->nnz, (int)NS, h->row_nz);
+  h->num_table_factor = ntf;
+  timing_mark(h, 5);
+  hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)(MA / kScanTile + 1)), dim3(kScanBS), 0, h->stream, MA,
+                     h->row_nz, (RowT *)c_row_ptr, h->c_rpt64, h->cls, h->tile_status, h->d_info);
+  timing_mark(h, 3);
+  if (into) {  // C's capacity is known: copy the slots now and read back once
+    timing_mark(h, 6);
+    launch_spec_compact(h, sizeof(V) == 8 ? NSP_F64 : NSP_F32, MA, NS, into->nnz, into->col, into->val);
+    timing_mark(h, 7);
+  }
+  NSP_HIP(h, hipGetLastError());
+  NSP_TRY(read_info(h));
+  if (hi.spec_over) return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz);  // a row outgrew its slot
+  if (hi.error) return set_error(h, (int)hi.error, "C has 2^31 or more entries; use NSP_IDX64");
+  h->stats.nprod = hi.nprod;
+  h->stats.nnz = hi.nnz;
+  h->stats.max_intprod = hi.max_intprod;
+  h->stats.max_row_nnz = hi.max_nz;
+  h->stats.n_sym_bins = kBins;
+  h->stats.n_num_bins = kBins;
+  for (int b = 0; b < kBins; b++) {
+    h->stats.sym_bin[b] = hi.count_sym[b];
+    h->stats.num_bin[b] = hi.count_num[b];
+  }
+  h->stats.sym_launches = 0;
+  h->stats.num_launches = 1;
+  h->stats.direct_mode = 2;
+  h->stats.fused = 1;
+  *c_nnz = hi.nnz;
+  h->fused_ns = NS;
+  h->fused_key[0] = A->col;
+  h->fused_key[1] = B->col;
+  h->plan_rows = MA;
+  h->plan_nnz = hi.nnz;
+  if (into) {
+    *filled = true;
+    h->timing_fused = h->timing;
+    h->timing_pending = h->timing;
+  } else {
+    h->fused_plan = true;
+  }
+  return NSP_OK;
+}
+
+int spgemm_fused_count(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                       void *c_row_ptr, int64_t *c_nnz, nsp_csr_t *into, bool *filled) {
+  NSP_TRY(check_csr_args(h, "A", A));
+  NSP_TRY(check_csr_args(h, "B", B));
+  if (vt != NSP_F64 && vt != NSP_F32) return set_error(h, NSP_ERR_INVALID, "unknown value type");
+  if (A->cols != B->rows) return set_error(h, NSP_ERR_INVALID, "A.cols != B.rows");
+  if (!c_row_ptr || !c_nnz) return set_error(h, NSP_ERR_INVALID, "c_row_ptr / c_nnz is NULL");
+  if (it == NSP_IDX32) {
+    if (A->nnz > INT32_MAX || B->nnz > INT32_MAX)
+      return set_error(h, NSP_ERR_OVERFLOW, "nnz >= 2^31 needs NSP_IDX64");
+    return vt == NSP_F64 ? fused_count_impl<int32_t, double>(h, A, B, c_row_ptr, c_nnz, into, filled)
+                         : fused_count_impl<int32_t, float>(h, A, B, c_row_ptr, c_nnz, into, filled);
+  }
+  if (it == NSP_IDX64)
+    return vt == NSP_F64 ? fused_count_impl<int64_t, double>(h, A, B, c_row_ptr, c_nnz, into, filled)
+                         : fused_count_impl<int64_t, float>(h, A, B, c_row_ptr, c_nnz, into, filled);
+  return set_error(h, NSP_ERR_INVALID, "unknown index type");
+}
+
+int spgemm_fused_fill(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                      nsp_csr_t *C) {
+  if (!h->fused_plan) return spgemm_numeric(h, it, vt, A, B, C);
+  h->fused_plan = false;
+  if (!A || !B || !C) return set_error(h, NSP_ERR_INVALID, "NULL matrix");
+  if (A->rows != h->plan_rows || A->col != h->fused_key[0] || B->col != h->fused_key[1])
+    return set_error(h, NSP_ERR_STATE, "A/B differ from the counting call");
+  if (C->nnz != h->plan_nnz) return set_error(h, NSP_ERR_INVALID, "C.nnz differs from the count");
+  if (C->nnz > 0 && (!C->col || !C->val)) return set_error(h, NSP_ERR_INVALID, "C.col/C.val is NULL");
+  timing_mark(h, 6);
+  launch_spec_compact(h, vt, A->rows, h->fused_ns, C->nnz, C->col, C->val);
+  timing_mark(h, 7);
+  h->timing_fused = h->timing;
+  h->timing_pending = h->timing;
+  NSP_HIP(h, hipGetLastError());
+  (void)it;
+  return NSP_OK;
 }
 
 // ---------------------------------------------------------------------------

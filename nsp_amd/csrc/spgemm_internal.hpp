@@ -7,6 +7,14 @@ int spgemm_symbolic(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *
                     void *c_row_ptr, int64_t *c_nnz);
 int spgemm_numeric(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
                    nsp_csr_t *C);
+// One-call product: C's row pointer and nnz (rows kept in the handle), then
+// C's columns and values; the pair is called back to back with the same A, B.
+// With `into` (C's buffers, capacity into->nnz) the rows may be copied
+// there already: *filled says so, and spgemm_fused_fill is then not called.
+int spgemm_fused_count(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                       void *c_row_ptr, int64_t *c_nnz, nsp_csr_t *into = nullptr, bool *filled = nullptr);
+int spgemm_fused_fill(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, const nsp_csr_t *B,
+                      nsp_csr_t *C);
 int spgemm_intprod(Handle *h, nsp_index_t it, const nsp_csr_t *A, const nsp_csr_t *B, int64_t *intprod,
                    int64_t *nprod);
 int check_csr(Handle *h, nsp_index_t it, const nsp_csr_t *X, int64_t *bad_row);
