@@ -113,6 +113,11 @@ struct Handle {
   bool timing_fused = false;
   int64_t fused_ns = 0;
   const void *fused_key[2] = {nullptr, nullptr};
+  // slot bin of the last one-call product that took the slot pass, with its
+  // buffers and sizes (-1: none)
+  int fc_bin = -1, fc_last = -1;
+  const void *fc_key[4] = {nullptr, nullptr, nullptr, nullptr};
+  int64_t fc_dims[4] = {0, 0, 0, 0};
   int64_t plan_rows = -1, plan_nnz = -1;
   const void *plan_a = nullptr, *plan_b = nullptr;
   int plan_sub = 16;

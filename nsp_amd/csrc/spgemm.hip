@@ -1964,30 +1964,48 @@ static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, v
   h->stats = nsp_stats_t{};
   h->have_plan = false;
   h->timing_pending = false;
+  // The slot size of the previous one-call product on the same buffers and
+  // sizes is taken again without a sample and without reading the row info
+  // back (no host synchronisation before the slot pass); a row that no
+  // longer fits is caught like any other.
+  const void *key[4] = {A->col, A->row_ptr, B->col, B->row_ptr};
+  const int64_t dims[4] = {MA, A->nnz, B->nnz, B->cols};
+  const bool cached = h->fc_bin >= 0 && std::equal(key, key + 4, h->fc_key) && std::equal(dims, dims + 4, h->fc_dims);
+  h->fc_bin = -1;
   NSP_TRY(run_rowinfo<RowT>(h, A, B));
-  const int64_t ns = std::min<int64_t>(MA, kSpecSample);
-  hipLaunchKernelGGL(k_sym_sample<kSymR>, dim3((unsigned)std::min<int64_t>((ns + 3) / 4, (int64_t)h->num_cu * 4)),
-                     dim3(256), 0, h->stream, MA, ns, h->a_rpt64, h->intprod, h->ab_b0, h->ab_len, B->col, h->d_info);
-  NSP_HIP(h, hipGetLastError());
-  timing_mark(h, 1);
-  NSP_TRY(read_info(h));
   const Info &hi = *h->h_info;
-  // the smallest wave-level numeric bin whose NS holds the sampled rows (a
-  // row past it is caught and redone in two passes)
   int bin = -1;
-  if (hi.count_sym[6] + hi.count_sym[7] + hi.count_sym[8] == 0 && hi.smp_max > 0)
-    for (int b = 0; b <= 5 && bin < 0; b++)
-      if (hi.smp_max <= kNumBound[b]) bin = b;
+  if (cached) {
+    bin = h->fc_last;
+    timing_mark(h, 1);
+  } else {
+    const int64_t ns = std::min<int64_t>(MA, kSpecSample);
+    hipLaunchKernelGGL(k_sym_sample<kSymR>, dim3((unsigned)std::min<int64_t>((ns + 3) / 4, (int64_t)h->num_cu * 4)),
+                       dim3(256), 0, h->stream, MA, ns, h->a_rpt64, h->intprod, h->ab_b0, h->ab_len, B->col,
+                       h->d_info);
+    NSP_HIP(h, hipGetLastError());
+    timing_mark(h, 1);
+    NSP_TRY(read_info(h));
+    // the smallest wave-level numeric bin whose NS holds the sampled rows (a
+    // row past it is caught and redone in two passes)
+    if (hi.count_sym[6] + hi.count_sym[7] + hi.count_sym[8] == 0 && hi.smp_max > 0)
+      for (int b = 0; b <= 5 && bin < 0; b++)
+        if (hi.smp_max <= kNumBound[b]) bin = b;
+  }
   const int64_t NS = bin >= 0 ? kNumBound[bin] : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t cbytes = al((size_t)MA * NS * 4), need = cbytes + al((size_t)MA * NS * sizeof(V));
-  if (bin < 0 || need > kSpecMaxBytes) return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz, Batch{}, true);
+  if (bin < 0 || need > kSpecMaxBytes) {
+    if (cached) NSP_TRY(read_info(h));  // the two-pass path starts from the row info
+    return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz, Batch{}, true);
+  }
   if (need > h->spec.bytes) {
     if (h->spec.base) NSP_HIP(h, hipFree(h->spec.base));
     h->spec.base = nullptr;
     h->spec.bytes = 0;
     if (hipMalloc(&h->spec.base, need) != hipSuccess) {
       (void)hipGetLastError();
+      if (cached) NSP_TRY(read_info(h));
       return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz, Batch{}, true);
     }
     h->spec.bytes = need;
@@ -2028,6 +2046,10 @@ static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, v
   h->stats.direct_mode = 2;
   h->stats.fused = 1;
   *c_nnz = hi.nnz;
+  h->fc_bin = bin;
+  h->fc_last = bin;
+  std::copy(key, key + 4, h->fc_key);
+  std::copy(dims, dims + 4, h->fc_dims);
   h->fused_ns = NS;
   h->fused_key[0] = A->col;
   h->fused_key[1] = B->col;

@@ -149,3 +149,42 @@ def test_repeated_calls_new_values_and_mixed_paths(nsp, orc):
         check_against_oracle(S_rep, S_rep, out, "f64", orc)
         C2 = h.spgemm(nsp.DeviceCSR.from_host(A), nsp.DeviceCSR.from_host(B))
         check_against_oracle(A, B, C2, "f64", orc)
+
+
+def test_cached_slot_size_with_a_changed_pattern(nsp, orc):
+    """A second one-call product on the same buffers and sizes reuses the
+    first call's slot size without a sample; when the pattern changed in
+    place so that a row no longer fits, the row is caught and the product
+    redone in two passes."""
+    print("case: fused cached slot size", flush=True)
+    import torch
+    rng = np.random.default_rng(95)
+    n, m = 8192, 5000
+    B = mixed_rows(m, 100_000, 8, 12, seed=96)
+
+    def rows(lens):
+        cols = np.concatenate([rng.choice(m, k, replace=False) for k in lens])
+        return gen._from_rows_cols(n, m, np.repeat(np.arange(n), lens), cols, rng.uniform(-1, 1, lens.sum()))
+
+    l1 = np.full(n, 3)
+    l2 = l1.copy()
+    l2[1] += 57
+    l2[2:59] -= 1
+    A1, A2 = rows(l1), rows(l2)
+    assert A1.nnz == A2.nnz
+    h = nsp.Handle()
+    dA, dB = nsp.DeviceCSR.from_host(A1), nsp.DeviceCSR.from_host(B)
+    C = h.spgemm_into(dA, dB)
+    assert h.stats()["fused"] == 1
+    check_against_oracle(A1, B, C, "f64", orc)
+    C = h.spgemm_into(dA, dB)  # same pattern: cached slot size
+    assert h.stats()["fused"] == 1 and h.stats()["host_syncs"] == 1
+    check_against_oracle(A1, B, C, "f64", orc)
+    d2 = nsp.DeviceCSR.from_host(A2)
+    dA.row_ptr.copy_(d2.row_ptr)
+    dA.col.copy_(d2.col)
+    dA.val.copy_(d2.val)
+    torch.cuda.synchronize()
+    C = h.spgemm_into(dA, dB)
+    assert h.stats()["fused"] == 0
+    check_against_oracle(A2, B, C, "f64", orc)
