@@ -143,6 +143,34 @@ class Handle:
         rpt, nnz = self.symbolic(A, B)
         return self.numeric(A, B, rpt, nnz)
 
+    def spgemm_once(self, A: DeviceCSR, B: DeviceCSR) -> DeviceCSR:
+        """One-call product with C allocated by the library through torch's
+        allocator (nsp_spgemm, the reference's nsp_spgemm_kernel_hash): the
+        symbolic pass is skipped when the row sample allows."""
+        torch = _torch()
+        _same_types(A, B)
+        dev = A.col.device
+        keep = {}
+
+        def _alloc(nbytes, user):
+            t = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=dev)
+            keep[t.data_ptr()] = t
+            return t.data_ptr()
+
+        def _free(ptr, user):
+            keep.pop(ptr, None)
+
+        alloc, free = _lib.ALLOC_FN(_alloc), _lib.FREE_FN(_free)
+        a, b = A.c_struct(), B.c_struct()
+        c = _lib.nsp_csr_t()
+        self._check(self.L.nsp_spgemm(self.h, A.index_type, A.value_type, C.byref(a), C.byref(b), C.byref(c),
+                                      C.cast(alloc, C.c_void_p), C.cast(free, C.c_void_p), None))
+        nnz, rows = int(c.nnz), int(c.rows)
+        rpt = keep[c.row_ptr].view(A.row_ptr.dtype)[:rows + 1]
+        col = keep[c.col].view(torch.int32)[:nnz] if c.col else torch.empty(0, dtype=torch.int32, device=dev)
+        val = keep[c.val].view(A.val.dtype)[:nnz] if c.val else torch.empty(0, dtype=A.val.dtype, device=dev)
+        return DeviceCSR(rpt, col, val, rows, int(c.cols))
+
     def spgemm_into(self, A: DeviceCSR, B: DeviceCSR, out: DeviceCSR | None = None,
                     capacity: int | None = None) -> DeviceCSR:
         """One-call product (nsp_spgemm_into) into `out`'s buffers (row

@@ -188,3 +188,14 @@ def test_cached_slot_size_with_a_changed_pattern(nsp, orc):
     C = h.spgemm_into(dA, dB)
     assert h.stats()["fused"] == 0
     check_against_oracle(A2, B, C, "f64", orc)
+
+
+def test_spgemm_once_allocates_through_torch(nsp, orc):
+    """Handle.spgemm_once: nsp_spgemm with C allocated through torch's
+    allocator, the one-call path underneath."""
+    print("case: spgemm_once", flush=True)
+    S = gen.stencil_27pt(18, seed=97)
+    h = nsp.Handle()
+    C = h.spgemm_once(nsp.DeviceCSR.from_host(S), nsp.DeviceCSR.from_host(S))
+    assert h.stats()["fused"] == 1
+    check_against_oracle(S, S, C, "f64", orc)
