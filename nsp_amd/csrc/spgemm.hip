@@ -1294,7 +1294,10 @@ __global__ __launch_bounds__(kHeavyBS) void k_num_heavy(int bin_lo, int bin_hi, 
 
 // ---------------------------------------------------------------------------
 // One-call product in one hash pass.  k_sym_sample counts the distinct
-// columns of up to kSpecSample evenly spaced rows (one wavefront per row,
+// columns of up to kSpecSample rows, one at a pseudo-random place in each of
+// kSpecSample equal blocks of rows (evenly spaced rows would all fall on one
+// face of a grid whose row count per plane divides the spacing), one
+// wavefront per row,
 // a 2048-slot LDS table; rows past 1024 products are left out, as such a
 // product does not take this path) and keeps the largest count.  When every
 // row of C is then expected to fit a numeric bin's NS entries, k_num_group
@@ -1319,7 +1322,9 @@ __global__ __launch_bounds__(256) void k_sym_sample(int64_t MA, int64_t nsample,
   int32_t *K = keys[w];
   const int64_t stride = MA / nsample;
   for (int64_t t = blockIdx.x * 4 + w; t < nsample; t += (int64_t)gridDim.x * 4) {
-    const int64_t r = t * stride;
+    uint32_t x = (uint32_t)t * 0x9E3779B1u;
+    x ^= x >> 15;
+    const int64_t r = t * stride + (int64_t)(((uint64_t)x * (uint64_t)stride) >> 32);
     if (intprod[r] > SH / 2) continue;
     clear_lds<64, SH + 64>(K, lane, kEmpty);
     wave_lds_sync();
@@ -1987,10 +1992,17 @@ static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, v
     timing_mark(h, 1);
     NSP_TRY(read_info(h));
     // the smallest wave-level numeric bin whose NS holds the sampled rows (a
-    // row past it is caught and redone in two passes)
-    if (hi.count_sym[6] + hi.count_sym[7] + hi.count_sym[8] == 0 && hi.smp_max > 0)
-      for (int b = 0; b <= 5 && bin < 0; b++)
-        if (hi.smp_max <= kNumBound[b]) bin = b;
+    // row past it is caught and redone in two passes), or the bin that holds
+    // the largest row's product count, which no row of C can outgrow, when
+    // it is at most one bin larger
+    if (hi.count_sym[6] + hi.count_sym[7] + hi.count_sym[8] == 0) {
+      int bs = -1, bp = -1;
+      for (int b = 5; b >= 0; b--) {
+        if (hi.smp_max > 0 && hi.smp_max <= kNumBound[b]) bs = b;
+        if (hi.max_intprod <= kNumBound[b]) bp = b;
+      }
+      bin = (bp >= 0 && (bs < 0 || bp <= bs + 1)) ? bp : bs;
+    }
   }
   const int64_t NS = bin >= 0 ? kNumBound[bin] : 0;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };

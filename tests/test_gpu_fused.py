@@ -67,9 +67,10 @@ def test_random_rows_each_bin_size(nsp, orc, rpw):
 
 
 def test_row_outgrowing_its_slot_falls_back(nsp, orc):
-    """The sample takes every other row; row 1 (unsampled) has far more
-    columns than the sampled rows, so it outgrows the slot: the product is
-    redone in two passes and still matches the oracle."""
+    """The sample takes one row in each block of eight (row 0 in the first
+    block); row 1 (never sampled) has far more columns than the sampled rows
+    and more products than the largest wave-level bin, so it outgrows the
+    slot: the product is redone in two passes and still matches the oracle."""
     print("case: fused fallback", flush=True)
     rng = np.random.default_rng(61)
     n = 8192
@@ -199,3 +200,28 @@ def test_spgemm_once_allocates_through_torch(nsp, orc):
     C = h.spgemm_once(nsp.DeviceCSR.from_host(S), nsp.DeviceCSR.from_host(S))
     assert h.stats()["fused"] == 1
     check_against_oracle(S, S, C, "f64", orc)
+
+
+def test_grid_aligned_rows_take_one_pass(nsp, orc):
+    """Stencil 32^3 times the aggregation prolongator: 32768 rows, so 1024
+    evenly spaced rows would all lie on the x = 0 face, whose C rows are
+    shorter than the interior's.  The sample and the product-count bound
+    still size the slot for every row: one pass, the oracle's C.  (P has
+    4096 columns, so the column-bitmap kernels are switched off.)"""
+    print("case: fused grid-aligned", flush=True)
+    S = gen.stencil_27pt(32, seed=98)
+    P = gen.aggregation(S.nrows, seed=99)
+    ref = orc.spgemm(S, P)
+    h = nsp.Handle()
+    h.set_param("narrow_mode", 0)
+    C = h.spgemm_into(nsp.DeviceCSR.from_host(S), nsp.DeviceCSR.from_host(P))
+    st = h.stats()
+    assert st["fused"] == 1 and st["sym_launches"] == 0, st
+    check_against_oracle(S, P, C, "f64", orc, ref=ref)
+    # P^T (A P) through nsp_galerkin: both products in one pass
+    Cg = h.galerkin(nsp.DeviceCSR.from_host(S), nsp.DeviceCSR.from_host(P))
+    assert h.stats()["fused"] == 1
+    trpt, tcol, tval = orc.transpose(P)
+    PT = gen.HostCSR(trpt, tcol, tval, P.ncols, P.nrows)
+    AP = gen.HostCSR(ref[0], ref[1], ref[2], S.nrows, P.ncols)
+    check_against_oracle(PT, AP, Cg, "f64", orc)
