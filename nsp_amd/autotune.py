@@ -4,7 +4,9 @@ The library's kernels come in shapes chosen per launch by handle parameters
 (nsp_set_param): rows per wavefront in the small and medium bins of each
 phase (the group size GS = 64 / rows), slots per intermediate product of the
 symbolic hash tables and per row entry of the numeric ones, blocks per CU of
-the grid-stride launches, and blocks in flight for heavy rows.  Which shape is fastest depends on the row-length distribution,
+the grid-stride launches, blocks in flight for heavy rows, and whether the
+numeric bins 0..5 of column-bitmap products run as one launch fed by
+row-bin work queues (with its ticket size and blocks per CU).  Which shape is fastest depends on the row-length distribution,
 so the tuner times the real product on the real input:
 
   * the symbolic parameters change only the symbolic pass and the numeric
@@ -32,11 +34,18 @@ import statistics
 
 DEFAULTS = {"sym_small_rows_per_wave": 4, "sym_rows_per_wave": 2, "sym_table_factor": 2,
             "num_small_rows_per_wave": 4, "num_rows_per_wave": 2, "num_table_factor": 2, "grid_cap": 8,
-            "heavy_blocks": 0}
+            "heavy_blocks": 0, "queue_mode": 1, "queue_chunk": 16, "queue_blocks": 4}
 
 SPACE_SYM = {"sym_small_rows_per_wave": (1, 2, 4, 8), "sym_rows_per_wave": (1, 2, 4), "sym_table_factor": (1, 2)}
 SPACE_NUM = {"num_small_rows_per_wave": (1, 2, 4, 8), "num_rows_per_wave": (1, 2, 4), "num_table_factor": (2, 4)}
 SPACE_GRID = {"grid_cap": (4, 8, 16)}
+# numeric bins 0..5 of column-bitmap products: per-bin launches (the shapes
+# above) or one launch fed by the row-bin work queues, with its ticket size
+# and blocks per CU
+SPACE_QUEUE = ({"queue_mode": 0}, {"queue_mode": 1, "queue_chunk": 8, "queue_blocks": 4},
+               {"queue_mode": 1, "queue_chunk": 16, "queue_blocks": 4},
+               {"queue_mode": 1, "queue_chunk": 32, "queue_blocks": 4},
+               {"queue_mode": 1, "queue_chunk": 16, "queue_blocks": 8})
 
 CACHE_ENV = "NSP_AMD_TUNE_CACHE"
 
@@ -117,7 +126,10 @@ def tune(handle, A, B, a_off=None, b_off=None, reps: int = 3, use_cache: bool = 
         apply(handle, best)
         return best
 
-    best = dict(DEFAULTS)
+    # the group shapes of the numeric bins are searched with the per-bin
+    # launches (the work-queue launch has fixed shapes), then the queue
+    # settings against them
+    best = dict(DEFAULTS, queue_mode=0)
     record = {}
     for space, phase in ((SPACE_SYM, 0), (SPACE_NUM, 1)):
         keys = list(space)
@@ -131,6 +143,15 @@ def tune(handle, A, B, a_off=None, b_off=None, reps: int = 3, use_cache: bool = 
         win = min(scores, key=scores.get)
         best.update(dict(zip(keys, win)))
         record["sym_ms" if phase == 0 else "num_ms"] = scores[win]
+    qscores = []
+    for q in SPACE_QUEUE:
+        apply(handle, dict(best, **q))
+        qscores.append(_measure(handle, run, reps)[1])
+        if log:
+            log(f"tune {q}: {qscores[-1]:.3f} ms")
+    qwin = min(range(len(SPACE_QUEUE)), key=qscores.__getitem__)
+    best.update(SPACE_QUEUE[qwin])
+    record["num_ms"] = qscores[qwin]
     totals = {}
     for cap in SPACE_GRID["grid_cap"]:
         apply(handle, dict(best, grid_cap=cap))

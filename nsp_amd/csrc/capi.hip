@@ -209,6 +209,15 @@ int nsp_set_param(nsp_handle_t hs, const char *name, int64_t value) {
     if (value < 1 || value > (int64_t(1) << 26))
       return nsp::set_error(h, NSP_ERR_INVALID, "coo_chunk_records must be in [1, 2^26]");
     h->coo_chunk = value;
+  } else if (!std::strcmp(name, "queue_mode")) {
+    if (value != 0 && value != 1) return nsp::set_error(h, NSP_ERR_INVALID, "queue_mode must be 0 or 1");
+    h->queue_mode = (int)value;
+  } else if (!std::strcmp(name, "queue_chunk")) {
+    if (value < 1 || value > 64) return nsp::set_error(h, NSP_ERR_INVALID, "queue_chunk must be in [1, 64]");
+    h->queue_chunk = (int)value;
+  } else if (!std::strcmp(name, "queue_blocks")) {
+    if (value < 1 || value > 16) return nsp::set_error(h, NSP_ERR_INVALID, "queue_blocks must be in [1, 16]");
+    h->queue_blocks = (int)value;
   } else if (!std::strcmp(name, "grid_cap")) {
     if (value < 1 || value > 64) return nsp::set_error(h, NSP_ERR_INVALID, "grid_cap must be in [1, 64]");
     h->grid_cap = (int)value;

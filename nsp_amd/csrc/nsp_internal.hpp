@@ -21,6 +21,9 @@ constexpr int64_t kSymBound[kBins] = {32, 64, 128, 256, 512, 1024, 2048, 16384, 
 // Upper bound (inclusive) of each numeric bin, on row nnz.
 constexpr int64_t kNumBound[kBins] = {16, 32, 64, 128, 256, 512, 2048, 4096, INT64_MAX};
 
+// Stride (in 32-bit words) between the work-queue heads in Info.
+constexpr int kQueueStride = 32;
+
 // Per-call counters living in device memory; mirrored to pinned host memory.
 struct Info {
   int64_t count_sym[16];
@@ -37,6 +40,10 @@ struct Info {
   int64_t nlong;  // rows binned by the long-row pass
   int64_t smp_max;    // one-call product: most distinct columns of a sampled row
   int64_t spec_over;  // one-call product: a row had more columns than its slot
+  // heads of the numeric row-bin work queues (one per XCD), each on a
+  // 128-byte line of its own: draws on one queue do not queue behind draws
+  // on another
+  alignas(128) uint32_t queue_num[8 * kQueueStride];
 };
 
 // A batch of independent products stored as concatenated rows: matrix b owns
@@ -78,6 +85,9 @@ struct Handle {
   int sym_table_factor = 2;  // symbolic slots per product bound in bins 0..3 (1 or 2)
   int narrow_mode = 1;       // 1: column bitmaps instead of hash tables when B has at most 4096 columns
   int fused_mode = 1;        // one-call products: 1 numeric pass into per-row slots + compaction when a sample allows
+  int queue_mode = 1;        // 1: narrow numeric bins 0..5 in one launch fed by row-bin work queues
+  int queue_chunk = 16;      // tiles per work-queue ticket
+  int queue_blocks = 4;      // blocks per CU of the work-queue launch
   int fp32_accumulate = 0;   // 1: FP32 values summed in FP32 where the kernel offers it (narrow products)
   int64_t coo_chunk = int64_t(1) << 21;  // records per streamed chunk of the COO reader
   bool timing = false;

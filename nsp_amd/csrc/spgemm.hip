@@ -50,7 +50,7 @@ __global__ void k_init(Info *info, uint64_t *tile_status, int64_t ntiles, int64_
   int64_t *w = reinterpret_cast<int64_t *>(info);
   constexpr int nw = sizeof(Info) / sizeof(int64_t);
   constexpr int wcap = (int)(offsetof(Info, cap) / sizeof(int64_t));
-  if (t < nw) w[t] = (t == wcap) ? cap : 0;
+  for (int64_t i = t; i < nw; i += (int64_t)gridDim.x * blockDim.x) w[i] = (i == wcap) ? cap : 0;
   for (int64_t i = t; i < ntiles; i += (int64_t)gridDim.x * blockDim.x) tile_status[i] = 0;
 }
 
@@ -817,34 +817,40 @@ __global__ __launch_bounds__(256) void k_sym_narrow(int bin_lo, int bin_hi, int6
   }
 }
 
-template <typename V, int GS, int NS, int R, bool O32, typename AccN = Acc>
-__global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int64_t direct_rows,
-                                                    const Info *__restrict__ info,
-                                                    const int32_t *__restrict__ perm,
-                                                    const int64_t *__restrict__ arpt,
-                                                    const int64_t *__restrict__ ab_b0,
-                                                    const V *__restrict__ aval,
-                                                    const int32_t *__restrict__ ab_len,
-                                                    const int32_t *__restrict__ bcol,
-                                                    const V *__restrict__ bval,
-                                                    const int64_t *__restrict__ crpt,
-                                                    int32_t *__restrict__ ccol,
-                                                    V *__restrict__ cval) {
+// LDS of a block of narrow numeric rows (RPB = 256 / GS rows).  Per bitmap
+// word: .x its bits, .y the set bits of the row's words before it.  Word j
+// of a row sits at wpos(j) = (j % WPL) * GS + j / WPL: the running count
+// gives lane g words [g*WPL, (g+1)*WPL), and at each step the group's lanes
+// read consecutive entries; rows are padded to 1152 bytes, so that the
+// groups of a wavefront start half the banks apart.
+template <typename V, int GS, int NS, typename AccN>
+struct NarrowNumLds {
+  static constexpr int RPB = 256 / GS;
+  alignas(16) uint2 wb[RPB][kNarrowWords + 16];
+  alignas(16) AccN vals[RPB][NS + GS];  // + one dummy slot per lane
+  EntryScratch<GS, V> ent[RPB];
+};
+
+// The rows of tiles first, first + step, ... (< end) of a row list of
+// `count` rows (perm[off + t], or t itself without perm), RPB = 256 / GS
+// rows a tile, row `slot` of each tile taken by this group of GS lanes,
+// with the group's LDS Wb / Vs / ent.
+template <typename V, int GS, int NS, int R, bool O32, typename AccN>
+__device__ __forceinline__ void num_narrow_rows(int tid, int64_t first, int64_t end, int64_t step, int64_t count,
+                                                int64_t off, const Info *__restrict__ info,
+                                                const int32_t *__restrict__ perm,
+                                                const int64_t *__restrict__ arpt,
+                                                const int64_t *__restrict__ ab_b0,
+                                                const V *__restrict__ aval,
+                                                const int32_t *__restrict__ ab_len,
+                                                const int32_t *__restrict__ bcol,
+                                                const V *__restrict__ bval,
+                                                const int64_t *__restrict__ crpt, int32_t *__restrict__ ccol,
+                                                V *__restrict__ cval, uint2 *Wb, AccN *Vs,
+                                                EntryScratch<GS, V> *ent) {
   constexpr int RPB = 256 / GS;
   constexpr int WPL = kNarrowWords / GS;
-  // per bitmap word: .x its bits, .y the set bits of the row's words before
-  // it.  Word j of a row sits at wpos(j) = (j % WPL) * GS + j / WPL: the
-  // running count gives lane g words [g*WPL, (g+1)*WPL), and at each step
-  // the group's lanes read consecutive entries; rows are padded to 1152
-  // bytes, so that the groups of a wavefront start half the banks apart.
-  __shared__ __align__(16) uint2 wb[RPB][kNarrowWords + 16];
-  __shared__ __align__(16) AccN vals[RPB][NS + GS];  // + one dummy slot per lane
-  __shared__ EntryScratch<GS, V> ent[RPB];
-  int64_t count, off;
-  bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
-  const int slot = threadIdx.x / GS, glane = threadIdx.x % GS;
-  uint2 *Wb = wb[slot];
-  AccN *Vs = vals[slot];
+  const int slot = tid / GS, glane = tid % GS;
   auto wpos = [](uint32_t j) -> uint32_t { return (j % WPL) * GS + j / WPL; };
   // adds a window's products into the value slots of their positions; with
   // `first` (the product that set its column's bit), that product also
@@ -866,14 +872,13 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
       if (first && first[u]) crow[rank] = (int32_t)k;
     }
   };
-  const XcdTiles xt((count + RPB - 1) / RPB);
   // the group's rows, one tile apart; the next row's A extent and first
   // chunk of entries are loaded while the current row is accumulated
   auto row_at = [&](int64_t tile) -> int32_t {
     const int64_t t = tile * RPB + slot;
-    return (tile < xt.end && t < count) ? (perm ? perm[off + t] : (int32_t)t) : -1;
+    return (tile < end && t < count) ? (perm ? perm[off + t] : (int32_t)t) : -1;
   };
-  int32_t rn = row_at(xt.first);
+  int32_t rn = row_at(first);
   int64_t an0 = 0, an1 = 0;
   FirstChunk<V> pre;
   if (rn >= 0) {
@@ -881,12 +886,12 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
     an1 = arpt[rn + 1];
     pre = load_first_chunk<true>(glane, an0, an1, ab_b0, ab_len, aval);
   }
-  for (int64_t tile = xt.first; rn >= 0; tile += xt.step) {
+  for (int64_t tile = first; rn >= 0; tile += step) {
     {
       const int32_t r = rn;
       const int64_t a0 = an0, a1 = an1;
       const FirstChunk<V> cur = pre;
-      rn = row_at(tile + xt.step);
+      rn = row_at(tile + step);
       if (rn >= 0) {
         an0 = arpt[rn];
         an1 = arpt[rn + 1];
@@ -909,7 +914,7 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
       bool so[R], sf[R];
       int windows = 0;
       row_products<GS, R, true, V, true, O32>(
-          glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
+          glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, ent,
           [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
 #pragma unroll
             for (int u = 0; u < R; u++) {
@@ -949,7 +954,7 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
       } else {
         // walk 2: each product into the slot of its column's position
         row_products<GS, R, true, V, true, O32>(
-            glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, &ent[slot],
+            glane, a0, a1, 0, GS, ab_b0, ab_len, aval, bcol, bval, ent,
             [&](const int32_t(&key)[R], const V(&prod)[R], const bool(&ok)[R], auto &&...) {
               accumulate(key, prod, ok, nullptr, nullptr);
             },
@@ -972,6 +977,156 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
       for (int j = glane; j < nz; j += GS) cval[c0 + j] = (V)Vs[j];
       wave_lds_sync();
     }
+  }
+}
+
+template <typename V, int GS, int NS, int R, bool O32, typename AccN = Acc>
+__global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int64_t direct_rows,
+                                                    const Info *__restrict__ info,
+                                                    const int32_t *__restrict__ perm,
+                                                    const int64_t *__restrict__ arpt,
+                                                    const int64_t *__restrict__ ab_b0,
+                                                    const V *__restrict__ aval,
+                                                    const int32_t *__restrict__ ab_len,
+                                                    const int32_t *__restrict__ bcol,
+                                                    const V *__restrict__ bval,
+                                                    const int64_t *__restrict__ crpt,
+                                                    int32_t *__restrict__ ccol,
+                                                    V *__restrict__ cval) {
+  __shared__ NarrowNumLds<V, GS, NS, AccN> L;
+  int64_t count, off;
+  bin_range(info->count_num, bin_lo, bin_hi, direct_rows, count, off);
+  const int slot = threadIdx.x / GS;
+  const XcdTiles xt((count + L.RPB - 1) / L.RPB);
+  num_narrow_rows<V, GS, NS, R, O32, AccN>(threadIdx.x, xt.first, xt.end, xt.step, count, off, info, perm, arpt, ab_b0, aval,
+                                           ab_len, bcol, bval, crpt, ccol, cval, L.wb[slot], L.vals[slot],
+                                           &L.ent[slot]);
+}
+
+// Narrow numeric rows of the wave-level bins 0..5 in one persistent launch
+// driven by row-bin work queues.  The tiles of each bin (RPB = 256 / GS rows
+// of its row list) are split into kQueues equal ranges, one per XCD; queue
+// x hands out tickets of `chunk` tiles over its ranges, the bins with the
+// longest rows first.  A block draws tickets from
+// the queue of its own XCD (blocks are dispatched to the XCDs in turn) and,
+// once that queue is spent, from the others in turn; it ends when all are
+// spent.  Each ticket's tiles are walked exactly as the per-bin kernel walks
+// its share, with the bin's group size (16 lanes up to 64 entries, 32 up to
+// 256, 64 up to 512), in one LDS pool that every bin's layout fits.  The
+// next ticket is drawn while the current one is worked, and one barrier per
+// ticket hands it to the block's waves (and frees the pool for the next
+// bin's layout).  Queue heads live in Info (zeroed by k_init), so the launch
+// is valid inside a captured graph.
+constexpr int kNumR = 4;  // B loads in flight per lane, numeric (at most)
+// Loads per lane of a numeric group kernel: a window of GS*R products is
+// held near the bin's row-length bound NS (a row has at least as many
+// products as entries); at least 2.
+constexpr int num_r(int GS, int NS) { return NS / GS >= kNumR ? kNumR : 2; }
+constexpr int kQueues = 8;
+constexpr int kQueueBins = 6;
+constexpr int queue_gs(int b) { return b <= 2 ? 16 : b <= 4 ? 32 : 64; }
+constexpr int queue_ns(int b) { return (int)kNumBound[b]; }
+template <typename V, typename AccN>
+constexpr size_t queue_pool_bytes() {
+  const size_t a = sizeof(NarrowNumLds<V, 16, 64, AccN>), b = sizeof(NarrowNumLds<V, 32, 256, AccN>),
+               c = sizeof(NarrowNumLds<V, 64, 512, AccN>);
+  return a > b ? (a > c ? a : c) : (b > c ? b : c);
+}
+
+// Bin and tile range [t0, t1) of ticket k of queue q, or bin -1 when the
+// queue has no ticket k.
+__device__ __forceinline__ int queue_ticket(const int64_t *cnt, int q, int64_t k, int chunk, int64_t &t0,
+                                            int64_t &t1) {
+  int bin = -1;
+#pragma unroll
+  for (int b = kQueueBins - 1; b >= 0; b--) {
+    const int rpb = 256 / queue_gs(b);
+    const int64_t nt = (cnt[b] + rpb - 1) / rpb;
+    const int64_t s = nt * q / kQueues, e = nt * (q + 1) / kQueues;
+    const int64_t nk = (e - s + chunk - 1) / chunk;
+    if (bin < 0) {
+      if (k < nk) {
+        bin = b;
+        t0 = s + k * chunk;
+        t1 = t0 + chunk < e ? t0 + chunk : e;
+      } else {
+        k -= nk;
+      }
+    }
+  }
+  return bin;
+}
+
+template <typename V, bool O32, typename AccN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(V) == 4 ? 4 : 1))) void k_num_narrow_queue(int chunk, Info *__restrict__ info,
+                                                          const int32_t *__restrict__ perm,
+                                                          const int64_t *__restrict__ arpt,
+                                                          const int64_t *__restrict__ ab_b0,
+                                                          const V *__restrict__ aval,
+                                                          const int32_t *__restrict__ ab_len,
+                                                          const int32_t *__restrict__ bcol,
+                                                          const V *__restrict__ bval,
+                                                          const int64_t *__restrict__ crpt,
+                                                          int32_t *__restrict__ ccol,
+                                                          V *__restrict__ cval) {
+  __shared__ __align__(16) unsigned char pool[queue_pool_bytes<V, AccN>()];
+  __shared__ uint32_t s_tk[2];
+  __shared__ int64_t cnt[kQueueBins], off[kQueueBins];  // rows of each bin and their place in perm
+  if (threadIdx.x == 0) {
+    int64_t o = 0;
+    for (int b = 0; b < kQueueBins; b++) {
+      cnt[b] = info->count_num[b];
+      off[b] = o;
+      o += cnt[b];
+    }
+  }
+  uint32_t *heads = info->queue_num;
+  int q = (int)(blockIdx.x % kQueues), tried = 1, it = 0;
+  if (threadIdx.x == 0) s_tk[0] = atomicAdd(&heads[q * kQueueStride], 1u);
+  __syncthreads();
+  // the ticket is the same in every lane: readfirstlane keeps it (and the
+  // bin switch below) scalar, so that the bins' code paths stay separate
+  // branches instead of being merged into one register budget
+  uint32_t tk = __builtin_amdgcn_readfirstlane(s_tk[0]);
+  for (;;) {
+    int64_t t0 = 0, t1 = 0;
+    const int bin = __builtin_amdgcn_readfirstlane(queue_ticket(cnt, q, (int64_t)tk, chunk, t0, t1));
+    if (bin < 0) {  // this queue is spent: the next one, until all are
+      if (tried == kQueues) break;
+      tried++;
+      q = (q + 1) % kQueues;
+      if (threadIdx.x == 0) s_tk[(it + 1) & 1] = atomicAdd(&heads[q * kQueueStride], 1u);
+      __syncthreads();
+      tk = __builtin_amdgcn_readfirstlane(s_tk[(it + 1) & 1]);
+      it++;
+      continue;
+    }
+    uint32_t nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(&heads[q * kQueueStride], 1u);
+#define QUEUE_BIN(B)                                                                                            \
+  case B: {                                                                                                     \
+    constexpr int GS = queue_gs(B), NS = queue_ns(B);                                                           \
+    int tid = (int)threadIdx.x;                                                                                 \
+    asm volatile("" : "+v"(tid));                                                                               \
+    auto *L = reinterpret_cast<NarrowNumLds<V, GS, NS, AccN> *>(pool);                                          \
+    const int slot = tid / GS;                                                                                  \
+    num_narrow_rows<V, GS, NS, num_r(GS, NS), O32, AccN>(tid, t0, t1, 1, cnt[B], off[B], info, perm, arpt, ab_b0, \
+                                                         aval, ab_len, bcol, bval, crpt, ccol, cval, L->wb[slot], \
+                                                         L->vals[slot], &L->ent[slot]);                          \
+  } break
+    switch (bin) {
+      QUEUE_BIN(0);
+      QUEUE_BIN(1);
+      QUEUE_BIN(2);
+      QUEUE_BIN(3);
+      QUEUE_BIN(4);
+      default: QUEUE_BIN(5);
+    }
+#undef QUEUE_BIN
+    if (threadIdx.x == 0) s_tk[(it + 1) & 1] = nxt;
+    __syncthreads();
+    tk = __builtin_amdgcn_readfirstlane(s_tk[(it + 1) & 1]);
+    it++;
   }
 }
 
@@ -1385,12 +1540,7 @@ constexpr int kSymR = 8;  // B loads in flight per lane, symbolic (at most)
 constexpr int sym_r(int GS, int SH) { return SH / GS >= kSymR ? kSymR : (SH / GS >= 4 ? 4 : 2); }
 // Symbolic table slots of a bin of product bound PB with the doubled table.
 constexpr int sym_slots2(int PB) { return PB <= 256 ? 2 * PB : PB; }
-constexpr int kNumR = 4;  // B loads in flight per lane, numeric (at most)
 
-// Loads per lane of a numeric group kernel: a window of GS*R products is
-// held near the bin's row-length bound NS (a row has at least as many
-// products as entries); at least 2.
-constexpr int num_r(int GS, int NS) { return NS / GS >= kNumR ? kNumR : 2; }
 
 // Launches the kernel of bin `bin` over bins [lo, bin] (rows of lower bins fit
 // its larger table), or over all rows in natural order when direct >= 0.
@@ -1552,6 +1702,34 @@ static int pack_ok(int64_t ncols, int slots) {
   int log = 0;
   while ((1 << log) < slots) log++;
   return ncols < (int64_t(1) << (32 - log)) ? 1 : 0;
+}
+
+// Whether the narrow numeric bins 0..5 run as one work-queue launch.
+static bool queue_ok(const Handle *h, int64_t ncols) {
+  return h->queue_mode && h->narrow_mode && ncols <= kNarrowCols;
+}
+
+// The work-queue launch over the narrow numeric bins 0..5 (rows listed by
+// bin in perm_num); a persistent grid of queue_blocks blocks per CU.
+template <typename V>
+static void launch_num_queue(Handle *h, const int64_t *arpt, const int64_t *ab_b0, const V *aval,
+                             const int32_t *ab_len, const int32_t *bcol, const V *bval, int32_t *ccol, V *cval,
+                             int64_t nnzB) {
+  const dim3 grid((unsigned)std::max(1, h->num_cu * h->queue_blocks));
+  const bool o32 = offsets32(nnzB);
+#define NQ(O, ACC)                                                                                            \
+  hipLaunchKernelGGL((k_num_narrow_queue<V, O, ACC>), grid, dim3(256), 0, h->stream, h->queue_chunk, h->d_info, \
+                     h->perm_num, arpt, ab_b0, aval, ab_len, bcol, bval, h->c_rpt64, ccol, cval)
+  if constexpr (sizeof(V) == 4) {
+    if (h->fp32_accumulate) {  // FP32 values summed in FP32
+      if (o32) NQ(true, float);
+      else NQ(false, float);
+      return;
+    }
+  }
+  if (o32) NQ(true, Acc);
+  else NQ(false, Acc);
+#undef NQ
 }
 
 template <typename V>
@@ -1897,6 +2075,28 @@ int spgemm_symbolic(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *
 
 template <typename V>
 static int numeric_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_csr_t *C) {
+  int nwave = 0;
+  for (int b = 0; b < kQueueBins; b++) nwave += h->num_count[b] > 0;
+  if (nwave >= 2 && queue_ok(h, B->cols)) {  // the wave-level bins in one work-queue launch
+    launch_scatter(h, A->rows, h->d_info->count_num, h->d_info->cursor_num, h->perm_num);
+    timing_mark(h, 4);
+    launch_num_queue<V>(h, h->a_rpt64, h->ab_b0, (const V *)A->val, h->ab_len, B->col, (const V *)B->val, C->col,
+                        (V *)C->val, B->nnz);
+    int n = 1;
+    for (int b = kBins - 1; b >= kQueueBins; b--)
+      if (h->num_count[b] > 0) {
+        launch_num_bin<V>(h, b, b, -1, h->num_count[b], h->a_rpt64, h->ab_b0, (const V *)A->val, h->ab_len,
+                          B->col, (const V *)B->val, C->col, (V *)C->val, B->cols, B->nnz);
+        n++;
+      }
+    h->stats.num_launches = n;
+    h->stats.direct_mode &= 1;
+    timing_mark(h, 5);
+    h->timing_fused = false;
+    h->timing_pending = h->timing;
+    NSP_HIP(h, hipGetLastError());
+    return NSP_OK;
+  }
   Launch L[kBins];
   const int n = plan_launches(h->num_count, A->rows, L);
   if (needs_row_list(L, n))
@@ -2175,7 +2375,13 @@ static int graph_record(Handle *g, const nsp_csr_t *A, const nsp_csr_t *B, const
   hipLaunchKernelGGL(k_scan<RowT>, dim3((unsigned)(MA / kScanTile + 1)), dim3(kScanBS), 0, g->stream, MA, g->row_nz,
                      (RowT *)C->row_ptr, g->c_rpt64, g->cls, g->tile_status, g->d_info);
   launch_scatter(g, MA, g->d_info->count_num, g->d_info->cursor_num, g->perm_num);
-  for (int b = 0; b < kBins; b++)
+  int b0 = 0;
+  if (queue_ok(g, B->cols)) {  // the wave-level bins in one work-queue launch
+    launch_num_queue<V>(g, g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len, B->col, (const V *)B->val, C->col,
+                        (V *)C->val, B->nnz);
+    b0 = kQueueBins;
+  }
+  for (int b = b0; b < kBins; b++)
     launch_num_bin<V>(g, b, b, -1, rows_hint(b, true), g->a_rpt64, g->ab_b0, (const V *)A->val, g->ab_len,
                       B->col, (const V *)B->val, C->col, (V *)C->val, B->cols, B->nnz);
   NSP_HIP(g, hipGetLastError());
@@ -2209,6 +2415,9 @@ int graph_create(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t *A, 
   g->sym_table_factor = h->sym_table_factor;
   g->narrow_mode = h->narrow_mode;
   g->fp32_accumulate = h->fp32_accumulate;
+  g->queue_mode = h->queue_mode;
+  g->queue_chunk = h->queue_chunk;
+  g->queue_blocks = h->queue_blocks;
   gp->rows = A->rows;
   gp->launch_stream = h->stream;
   auto fail = [&](int s) {
@@ -2351,6 +2560,9 @@ static int noalloc_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
   g.sym_table_factor = h->sym_table_factor;
   g.narrow_mode = h->narrow_mode;
   g.fp32_accumulate = h->fp32_accumulate;
+  g.queue_mode = h->queue_mode;
+  g.queue_chunk = h->queue_chunk;
+  g.queue_blocks = h->queue_blocks;
   g.d_info = reinterpret_cast<Info *>(base);
   g.ws.base = base + kInfoBytes;
   g.ws.bytes = kNoAllocScratch - kInfoBytes;  // >= need: ensure_workspace only carves it

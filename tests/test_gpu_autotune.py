@@ -51,3 +51,6 @@ def test_tune_batched(orc, tmp_path, monkeypatch):
     assert torch.equal(C.col, ref.col) and torch.equal(C.row_ptr, ref.row_ptr)
     assert float((C.val - ref.val).abs().max()) <= 1e-4
     assert set(p) == set(autotune.DEFAULTS)
+    # the numeric launch form is one of the searched ones (per-bin, or the
+    # work-queue launch with one of its ticket sizes and grids)
+    assert any(all(p[k] == v for k, v in q.items()) for q in autotune.SPACE_QUEUE), p

@@ -224,3 +224,39 @@ def test_batched_full_size_4096(nsp, orc):
     assert rel <= TOL["f32"], rel
     del C, g
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_work_queue_launch_every_bin(nsp, orc, dtype):
+    """The numeric bins 0..5 of a column-bitmap product in one launch fed by
+    the row-bin work queues (queue_mode 1, the default) against the per-bin
+    launches (queue_mode 0) and the oracle: rows in every numeric bin up to
+    the block bins, several ticket sizes and grid sizes (1 block per CU: the
+    blocks steal from the other XCDs' queues), eager and graph-captured."""
+    print("case: work queue", dtype, flush=True)
+    from tests.test_gpu_spgemm import mixed_rows
+    A = to_dtype(mixed_rows(3000, 2500, 1, 70, seed=31), dtype)
+    B = to_dtype(mixed_rows(2500, 4000, 1, 12, seed=32), dtype)
+    ref = orc.spgemm(A, B)
+    dA, dB = nsp.DeviceCSR.from_host(A, dtype), nsp.DeviceCSR.from_host(B, dtype)
+    h0 = nsp.Handle()
+    h0.set_param("queue_mode", 0)
+    C0 = h0.spgemm(dA, dB)
+    nb = h0.stats()["num_bin"]
+    assert all(nb[b] > 0 for b in range(6)), nb
+    check_against_oracle(A, B, C0, dtype, orc, ref=ref)
+    for chunk, blocks in ((4, 4), (1, 1), (3, 2), (64, 4)):
+        h = nsp.Handle()
+        h.set_param("queue_chunk", chunk)
+        h.set_param("queue_blocks", blocks)
+        C = h.spgemm(dA, dB)
+        assert h.stats()["num_launches"] == 1 + sum(nb[b] > 0 for b in range(6, 9))
+        # bin 5 runs 64 lanes a row here and 32 per bin, so sums may differ in
+        # their last bits: structure bit-exact, values against the oracle
+        np.testing.assert_array_equal(C.row_ptr.cpu().numpy(), C0.row_ptr.cpu().numpy())
+        np.testing.assert_array_equal(C.col.cpu().numpy(), C0.col.cpu().numpy())
+        check_against_oracle(A, B, C, dtype, orc, ref=ref)
+    g = nsp.Handle().graph(dA, dB, capacity=int(ref[0][-1]))
+    for rep in range(3):
+        g.launch()
+        check_against_oracle(A, B, g.result(), dtype, orc, ref=ref)
