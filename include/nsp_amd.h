@@ -117,6 +117,13 @@ int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
  *                  counts it, a scan gives C's row pointer, and a copy packs
  *                  the slots into C (a row that outgrows its slot sends the
  *                  product back to two passes); 0: always two passes
+ *   "queue_mode"   1 (default): with the column bitmaps, the numeric rows of
+ *                  up to 512 entries run as one persistent launch whose
+ *                  blocks draw tiles of rows from row-bin work queues (one
+ *                  per XCD, longest rows first, idle blocks take from the
+ *                  other XCDs' queues); 0: one launch per bin
+ *   "queue_chunk"  tiles of rows per work-queue ticket (1..64, default 16)
+ *   "queue_blocks" blocks per CU of the work-queue launch (1..16, default 4)
  *   "sym_table_factor"   symbolic hash slots per intermediate product in the
  *                  bins of up to 256 products (1, 2; default 2: rows whose
  *                  products are mostly distinct keep the table half empty)
