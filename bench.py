@@ -266,7 +266,7 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
            "graph_GFLOPs": round(2.0 * nprod / t_graph / 1e6, 1), "autotuned": tuned,
            "phases_ms_eager": {"binning": round(ph["t_bin_ms"], 3), "symbolic": round(ph["t_sym_ms"], 3),
                                "scan": round(ph["t_scan_ms"], 3), "numeric": round(ph["t_num_ms"], 3)},
-           "roofline": roofline_hbm(num_b, ph["t_num_ms"], "numeric pass (column-bitmap kernels), eager",
+           "roofline": roofline_hbm(num_b, ph["t_num_ms"], "numeric pass (column-bitmap kernels; bins 0-5 in one work-queue launch when the tuner picks it), eager",
                                     measured_traffic("batched")),
            "two_pass_bytes": sym_b + scan_b + num_b,
            "two_pass_frac_graph": round((sym_b + scan_b + num_b) / (t_graph / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
