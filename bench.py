@@ -270,6 +270,19 @@ def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int =
                                     measured_traffic("batched")),
            "two_pass_bytes": sym_b + scan_b + num_b,
            "two_pass_frac_graph": round((sym_b + scan_b + num_b) / (t_graph / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
+    # the same graph with FP32 accumulation (the arithmetic the batch
+    # configuration names): FP32 LDS slots, ds_add_f32
+    h32 = nsp_amd.Handle()
+    autotune.apply(h32, tuned)
+    h32.set_param("fp32_accumulate", 1)
+    g32 = h32.graph(dX, dX, capacity=nnz, a_off=d_off, b_off=d_off)
+    t32 = _timed(g32.launch, 2, 1)
+    C32 = g32.result()
+    out["fp32_accumulate"] = {
+        "graph_ms": round(t32, 3), "structure_equal": bool(torch.equal(C32.col, Cg.col)),
+        "max_abs_diff_vs_f64_accumulate": float((C32.val - Cg.val).abs().max()) if nnz else 0.0}
+    g32.close()
+    del g32, C32
     if not loop:
         return out
 
