@@ -43,6 +43,7 @@ __global__ void k_shrink_init(int64_t range, int64_t base, int64_t l, uint8_t *_
 
 __global__ void k_shrink_mark(int64_t nnz, const int32_t *__restrict__ col, int64_t base, int64_t range,
                               uint8_t *__restrict__ flags, uint32_t *__restrict__ bad) {
+#pragma unroll 4
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t x = (int64_t)col[e] - base;
@@ -55,6 +56,7 @@ __global__ void k_shrink_mark(int64_t nnz, const int32_t *__restrict__ col, int6
 // maps to -1 instead of indexing past the position array.
 __global__ void k_shrink_remap(int64_t nnz, const int32_t *__restrict__ col, int64_t base, int64_t range,
                                const int32_t *__restrict__ pos, int32_t *__restrict__ new_col) {
+#pragma unroll 4
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t x = (int64_t)col[e] - base;
@@ -72,6 +74,7 @@ __global__ void k_shrink_perm(int64_t range, int64_t base, const uint8_t *__rest
 template <typename RowT>
 __global__ void k_gather_len(int64_t nr, const int32_t *__restrict__ rows, int64_t row_offset, int64_t MB,
                              const RowT *__restrict__ brpt, int64_t *__restrict__ len, uint32_t *bad) {
+#pragma unroll 4
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nr;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t g = (int64_t)rows[i] + row_offset;
@@ -84,7 +87,16 @@ __global__ void k_gather_len(int64_t nr, const int32_t *__restrict__ rows, int64
   }
 }
 
-// One wavefront per gathered row: coalesced copy of columns (shifted) and values.
+// A block copies kGatherRows gathered rows at a time as one run of output
+// entries: the rows' source and destination starts are staged in LDS, then
+// consecutive threads copy consecutive output entries (coalesced reads and
+// writes across row boundaries; a stencil share's rows hold only ~27
+// entries), each finding its row by a binary search over the staged
+// destination starts.  Entries are taken only inside their source row, so a
+// destination row pointer that disagrees with B's row lengths cannot read
+// outside B; rows outside B copy nothing.
+constexpr int kGatherRows = 64;
+
 template <typename RowT, typename V>
 __global__ __launch_bounds__(256) void k_gather_fill(int64_t nr, const int32_t *__restrict__ rows,
                                                      int64_t row_offset, int64_t MB,
@@ -93,16 +105,38 @@ __global__ __launch_bounds__(256) void k_gather_fill(int64_t nr, const int32_t *
                                                      const V *__restrict__ bval, int64_t col_shift,
                                                      const RowT *__restrict__ orpt,
                                                      int32_t *__restrict__ ocol, V *__restrict__ oval) {
-  const int lane = threadIdx.x & 63;
-  for (int64_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < nr; i += (int64_t)gridDim.x * 4) {
-    const int64_t g = (int64_t)rows[i] + row_offset;
-    if (g < 0 || g >= MB) continue;
-    const int64_t s0 = (int64_t)brpt[g], s1 = (int64_t)brpt[g + 1];
-    const int64_t d0 = (int64_t)orpt[i];
-    for (int64_t q = lane; q < s1 - s0; q += 64) {
-      ocol[d0 + q] = (int32_t)((int64_t)bcol[s0 + q] - col_shift);
-      oval[d0 + q] = bval[s0 + q];
+  __shared__ int64_t s_dst[kGatherRows + 1];  // destination start of each row, then the run's end
+  __shared__ int64_t s_src[kGatherRows];      // source start of each row (-1: row outside B)
+  __shared__ int64_t s_len[kGatherRows];      // source length of each row
+  for (int64_t i0 = blockIdx.x * (int64_t)kGatherRows; i0 < nr; i0 += (int64_t)gridDim.x * kGatherRows) {
+    const int nrow = (int)(nr - i0 < kGatherRows ? nr - i0 : kGatherRows);
+    if ((int)threadIdx.x <= nrow) {
+      const int64_t i = i0 + threadIdx.x;
+      s_dst[threadIdx.x] = (int64_t)orpt[i];  // i <= nr: the row pointer has nr + 1 entries
+      if ((int)threadIdx.x < nrow) {
+        const int64_t g = (int64_t)rows[i] + row_offset;
+        const bool in = g >= 0 && g < MB;
+        s_src[threadIdx.x] = in ? (int64_t)brpt[g] : -1;
+        s_len[threadIdx.x] = in ? (int64_t)brpt[g + 1] - (int64_t)brpt[g] : 0;
+      }
     }
+    __syncthreads();
+    const int64_t D0 = s_dst[0], D1 = s_dst[nrow];
+#pragma unroll 4
+    for (int64_t e = D0 + threadIdx.x; e < D1; e += blockDim.x) {
+      int lo = 0, hi = nrow;  // s_dst[lo] <= e < s_dst[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_dst[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      const int64_t k = e - s_dst[lo];
+      if (s_src[lo] >= 0 && k >= 0 && k < s_len[lo]) {
+        ocol[e] = (int32_t)((int64_t)bcol[s_src[lo] + k] - col_shift);
+        oval[e] = bval[s_src[lo] + k];
+      }
+    }
+    __syncthreads();  // the staged starts are rewritten by the next run
   }
 }
 
@@ -223,7 +257,7 @@ int gather_rows_fill(Handle *h, nsp_index_t it, nsp_value_t vt, const nsp_csr_t 
   if (!B || !out || !out->row_ptr || nr < 0 || (nr > 0 && !rows))
     return set_error(h, NSP_ERR_INVALID, "nsp_gather_rows_fill: bad arguments");
   if (nr == 0 || out->nnz == 0) return NSP_OK;
-  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nr + 3) / 4, (int64_t)h->num_cu * 16));
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>((nr + kGatherRows - 1) / kGatherRows, (int64_t)h->num_cu * 8));
   hipStream_t st = h->stream;
 #define GATHER(RT, VT)                                                                                \
   hipLaunchKernelGGL((k_gather_fill<RT, VT>), dim3(g), dim3(256), 0, st, nr, rows, row_offset, B->rows, \
