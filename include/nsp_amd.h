@@ -97,7 +97,7 @@ const char *nsp_status_string(int status);
 const char *nsp_last_error(nsp_handle_t h);
 int nsp_get_stats(nsp_handle_t h, nsp_stats_t *out);
 /* Tuning knobs (nsp_amd.autotune picks them per workload):
- *   "grid_cap"     blocks per CU for persistent launches (1..64)
+ *   "grid_cap"     blocks per CU of the grid-stride launches (1..64, default 32)
  *   "sym_small_rows_per_wave"  rows per wavefront in the symbolic bins of
  *                  up to 128 products (1, 2, 4, 8)
  *   "sym_rows_per_wave"  rows per wavefront in the symbolic bins of

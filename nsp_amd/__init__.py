@@ -84,6 +84,7 @@ class Handle:
         s = stream if stream is not None else torch.cuda.current_stream()
         self.stream = s
         self._check(self.L.nsp_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+        self.params = {}  # parameters set on this handle (name -> value)
 
     def _check(self, status: int):
         if status != _lib.NSP_OK:
@@ -103,6 +104,7 @@ class Handle:
 
     def set_param(self, name: str, value: int):
         self._check(self.L.nsp_set_param(self.h, name.encode(), int(value)))
+        self.params[name] = int(value)
 
     def stats(self) -> dict:
         st = _lib.nsp_stats_t()

@@ -33,12 +33,12 @@ import os
 import statistics
 
 DEFAULTS = {"sym_small_rows_per_wave": 4, "sym_rows_per_wave": 2, "sym_table_factor": 2,
-            "num_small_rows_per_wave": 4, "num_rows_per_wave": 2, "num_table_factor": 2, "grid_cap": 8,
+            "num_small_rows_per_wave": 4, "num_rows_per_wave": 2, "num_table_factor": 2, "grid_cap": 32,
             "heavy_blocks": 0, "queue_mode": 1, "queue_chunk": 16, "queue_blocks": 4}
 
 SPACE_SYM = {"sym_small_rows_per_wave": (1, 2, 4, 8), "sym_rows_per_wave": (1, 2, 4), "sym_table_factor": (1, 2)}
 SPACE_NUM = {"num_small_rows_per_wave": (1, 2, 4, 8), "num_rows_per_wave": (1, 2, 4), "num_table_factor": (2, 4)}
-SPACE_GRID = {"grid_cap": (4, 8, 16)}
+SPACE_GRID = {"grid_cap": (8, 16, 32)}
 # numeric bins 0..5 of column-bitmap products: per-bin launches (the shapes
 # above) or one launch fed by the row-bin work queues, with its ticket size
 # and blocks per CU

@@ -582,7 +582,7 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
     // longer rows go to one for tier 3 (up to 1024 entries, 64 lanes), whose
     // longer rows go to the search tier (any length).  The small first tier
     // keeps the LDS per block low (occupancy) for the common short C rows.
-    const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 8));
+    const int ga = (int)std::max<int64_t>(1, std::min<int64_t>((MA + 7) / 8, (int64_t)h->num_cu * 32));
     hipLaunchKernelGGL((k_grad_a<V, RowT, 32, 32>), dim3(ga), dim3(256), 0, st, MA, (const int32_t *)nullptr,
                        (const unsigned int *)nullptr, (const RowT *)A->row_ptr, h->ab_b0, h->ab_len, B->col,
                        (const V *)B->val, (const RowT *)C->row_ptr, C->col, dc, da, (const V *)A->val,
@@ -607,7 +607,7 @@ static int backward_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, cons
     // lanes per B entry from the mean A^T row length
     const int64_t avg = A->cols > 0 ? (A->nnz + A->cols - 1) / A->cols : 1;
     const int g = (int)std::max<int64_t>(
-        1, std::min<int64_t>((B->rows + kGradBRows - 1) / kGradBRows, (int64_t)h->num_cu * 8));
+        1, std::min<int64_t>((B->rows + kGradBRows - 1) / kGradBRows, (int64_t)h->num_cu * 32));
 #define GRAD_B(SG)                                                                                            \
   if (contrib)                                                                                                \
     hipLaunchKernelGGL((k_grad_b_sum<V, RowT, SG>), dim3(g), dim3(256), 0, st, B->rows,                       \

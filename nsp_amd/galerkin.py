@@ -33,10 +33,19 @@ class GalerkinPlan:
         AP = self.h.spgemm(A, P)
         torch.index_select(P.val, 0, self.pt_perm, out=self.PT.val)
         C = self.h.spgemm(self.PT, AP)
-        self.g_ap = self.h.graph(A, P, capacity=AP.nnz)
+        # a recorded graph launches every bin's kernels whether or not the
+        # pattern puts rows in them; the plan's graphs are recorded with 8
+        # blocks per CU so that the empty launches stay short (a handle on
+        # the same stream with the caller's parameters)
+        gh = Handle(stream=self.h.stream)
+        for k, v in self.h.params.items():
+            gh.set_param(k, v)
+        gh.set_param("grid_cap", min(8, self.h.params.get("grid_cap", 8)))
+        self.g_ap = gh.graph(A, P, capacity=AP.nnz)
         self.AP = DeviceCSR(self.g_ap.C.row_ptr, self.g_ap.C.col[:AP.nnz], self.g_ap.C.val[:AP.nnz],
                             A.nrows, P.ncols)
-        self.g_c = self.h.graph(self.PT, self.AP, capacity=C.nnz)
+        self.g_c = gh.graph(self.PT, self.AP, capacity=C.nnz)
+        self._gh = gh
         self.nnz_ap, self.nnz = AP.nnz, C.nnz
 
     def launch(self):
