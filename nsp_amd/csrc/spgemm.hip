@@ -1517,9 +1517,10 @@ __global__ __launch_bounds__(256) void k_spec_compact(int64_t MA, int64_t NS, in
 
 // ---------------------------------------------------------------------------
 // host-side launch tables
-// Blocks per CU of the block-per-row kernels at most: their rows are long and
-// few, and more blocks only lengthened the R-MAT numeric pass (0.66 -> 0.70 s
-// at 32).
+// Blocks per CU of the numeric block-per-row kernels at most: their rows are
+// long and few, and more blocks lengthened the R-MAT numeric pass (0.66 ->
+// 0.70 s at 32; its symbolic pass, on the other hand, went from 0.184 to
+// 0.176 s, so the symbolic block kernels take the handle's grid size).
 constexpr int kBlockGridCap = 8;
 
 static int grid_for(const Handle *h, int64_t items, int per_block, int blocks_per_cu) {
@@ -1668,7 +1669,7 @@ static void launch_sym_bin(Handle *h, int bin, int lo, int64_t direct, int64_t c
     else { SYM_GROUP_T(GS, PB, PB); }                                                              \
   } while (0)
 #define SYM_BLOCK(BS, SH)                                                                          \
-  hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, std::min(cap, kBlockGridCap))), dim3(BS), 0, \
+  hipLaunchKernelGGL((k_sym_block<BS, SH, kSymR>), dim3(grid_for(h, count, 1, cap)), dim3(BS), 0, \
                      st, lo, bin, direct, info, perm, arpt, ab_b0, ab_len, bcol, rnz)
   // wave-level bins: 1, 2 or 4 rows per wavefront (GS = 64, 32, 16 lanes per
   // row; more rows in flight per CU against fewer lanes per row), set by
