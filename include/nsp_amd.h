@@ -61,7 +61,9 @@ typedef struct nsp_handle_s *nsp_handle_t;
 typedef struct {
   int64_t nprod;            /* sum_i sum_{k in A_i} nnz(B_k) */
   int64_t nnz;              /* nnz(C) */
-  int64_t max_intprod;      /* largest intermediate-product count of a row */
+  int64_t max_intprod;      /* largest intermediate-product count of a row
+                             * (-1 after a one-call product that reused the
+                             * previous call's slot size: it sums no rows) */
   int64_t max_row_nnz;      /* largest nnz of a C row */
   int64_t sym_bin[16];      /* rows per symbolic bin */
   int64_t num_bin[16];      /* rows per numeric bin */

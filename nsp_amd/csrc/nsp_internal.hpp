@@ -139,6 +139,7 @@ struct Handle {
   int32_t *row_nz = nullptr, *perm_sym = nullptr, *perm_num = nullptr;
   uint8_t *cls = nullptr;
   uint64_t *tile_status = nullptr;
+  unsigned long long *ext_partial = nullptr;  // k_extents' per-block product sums (light row info)
   int64_t ntiles = 0;
 };
 

@@ -103,8 +103,10 @@ __global__ __launch_bounds__(256) void k_extents(int64_t nnzA, const RowT *__res
                                                  const RowT *__restrict__ brpt,
                                                  const int64_t *__restrict__ a_row_off,
                                                  const int64_t *__restrict__ b_row_off, int64_t count,
-                                                 int64_t *__restrict__ ab_b0, int32_t *__restrict__ ab_len) {
+                                                 int64_t *__restrict__ ab_b0, int32_t *__restrict__ ab_len,
+                                                 unsigned long long *__restrict__ partial) {
   __shared__ int64_t s_mat[2];
+  unsigned long long lsum = 0;  // this thread's intermediate products (for `partial`)
   const int t = threadIdx.x;
   constexpr int64_t TILE = 256 * kExtU;
   // each block takes a contiguous run of tiles
@@ -153,9 +155,17 @@ __global__ __launch_bounds__(256) void k_extents(int64_t nnzA, const RowT *__res
       if (v[u]) {
         ab_b0[p[u]] = b0[u];
         ab_len[p[u]] = (int32_t)(b1[u] - b0[u]);
+        lsum += (unsigned long long)(b1[u] - b0[u]);
       }
     }
     if (a_row_off) __syncthreads();  // s_mat is rewritten by the next tile
+  }
+  if (partial) {  // the block's intermediate products, one slot per block (no atomics)
+    __shared__ unsigned long long s_red[4];
+    lsum = group_sum<64>(lsum);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = lsum;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = s_red[0] + s_red[1] + s_red[2] + s_red[3];
   }
 }
 
@@ -318,6 +328,27 @@ __global__ __launch_bounds__(256) void k_rowinfo_long(const RowT *__restrict__ a
   if (threadIdx.x == 0 && my_sum) {
     atomicAdd((unsigned long long *)&info->nprod, my_sum);
     atomicMax((unsigned long long *)&info->max_intprod, my_max);
+  }
+}
+
+// The part of the row info a one-call product needs when it reuses the
+// previous call's slot size: A's row pointer widened to int64, and the total
+// of the intermediate products from k_extents' per-block sums (block 0 adds
+// them up; no per-row sums, bins or long-row list).
+template <typename RowT>
+__global__ __launch_bounds__(256) void k_rowinfo_light(int64_t MA, const RowT *__restrict__ arpt,
+                                                       const unsigned long long *__restrict__ partial, int npart,
+                                                       int64_t *__restrict__ arpt64, Info *info) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = t; i <= MA; i += stride) arpt64[i] = (int64_t)arpt[i];
+  if (blockIdx.x == 0) {
+    unsigned long long s = 0;
+    for (int q = threadIdx.x; q < npart; q += blockDim.x) s += partial[q];
+    __shared__ unsigned long long s_red[4];
+    s = group_sum<64>(s);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) info->nprod = (int64_t)(s_red[0] + s_red[1] + s_red[2] + s_red[3]);
   }
 }
 
@@ -1905,11 +1936,14 @@ int collect_timing(Handle *h) {
 
 // ---------------------------------------------------------------------------
 // Bytes of the product workspace for A with MA rows and nnzA entries.
+// Per-block product sums k_extents leaves for the light row info, at most.
+constexpr int kExtPartials = 4096;
+
 static size_t workspace_bytes(int64_t MA, int64_t nnzA) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const int64_t ntiles = MA / kScanTile + 2;
   return al((MA + 1) * 8) * 3 + al(nnzA * 8 + 8) + al(nnzA * 4 + 4) + al(MA * 4) * 3 + al(MA + 1) +
-         al(ntiles * 8);
+         al(ntiles * 8) + al(kExtPartials * 8);
 }
 
 static int ensure_workspace(Handle *h, int64_t MA, int64_t nnzA) {
@@ -1940,6 +1974,7 @@ static int ensure_workspace(Handle *h, int64_t MA, int64_t nnzA) {
   h->perm_num = (int32_t *)take(MA * 4);
   h->cls = (uint8_t *)take(MA + 1);
   h->tile_status = (uint64_t *)take(ntiles * 8);
+  h->ext_partial = (unsigned long long *)take(kExtPartials * 8);
   h->ntiles = ntiles;
   return NSP_OK;
 }
@@ -1978,7 +2013,8 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
     const int ge = (int)std::max<int64_t>(
         1, std::min<int64_t>((nnzA + 256 * kExtU - 1) / (256 * kExtU), (int64_t)h->num_cu * 8));
     hipLaunchKernelGGL(k_extents<RowT>, dim3(ge), dim3(256), 0, h->stream, nnzA, (const RowT *)A->row_ptr, A->col,
-                       (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->ab_b0, h->ab_len);
+                       (const RowT *)B->row_ptr, bt.a_off, bt.b_off, bt.count, h->ab_b0, h->ab_len,
+                       (unsigned long long *)nullptr);
   }
   // two blocks per CU: every block ends with a few global atomics on the
   // same counters, which serialise (8 blocks per CU: 58 us on the 524 K-row
@@ -1995,6 +2031,43 @@ static int run_rowinfo(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, const 
   h->ext_key[1] = A->row_ptr;
   h->ext_key[2] = B->row_ptr;
   h->ext_key[3] = bt.a_off;
+  h->ext_rows = MA;
+  h->ext_nnz = A->nnz;
+  h->ext_nprod = -1;
+  h->poff_valid = false;
+  return NSP_OK;
+}
+
+// Row info of a one-call product that reuses the previous call's slot size
+// (k_rowinfo_light in place of k_rowinfo: the slot pass walks every row in
+// natural order and needs neither bins nor per-row product counts).
+template <typename RowT>
+static int run_rowinfo_light(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B) {
+  const int64_t MA = A->rows;
+  NSP_TRY(ensure_workspace(h, MA, A->nnz));
+  timing_mark(h, 0);
+  hipLaunchKernelGGL(k_init, dim3(std::max<int64_t>(1, std::min<int64_t>((h->ntiles + 255) / 256, 1024))),
+                     dim3(256), 0, h->stream, h->d_info, h->tile_status, h->ntiles, INT64_MAX);
+  const int64_t nnzA = A->nnz;
+  unsigned long long *partial = h->ext_partial;  // one sum per k_extents block
+  int ge = 0;
+  if (nnzA > 0) {
+    ge = (int)std::max<int64_t>(
+        1, std::min<int64_t>({(nnzA + 256 * kExtU - 1) / (256 * kExtU), (int64_t)h->num_cu * 8,
+                              (int64_t)kExtPartials}));
+    hipLaunchKernelGGL(k_extents<RowT>, dim3(ge), dim3(256), 0, h->stream, nnzA, (const RowT *)A->row_ptr, A->col,
+                       (const RowT *)B->row_ptr, (const int64_t *)nullptr, (const int64_t *)nullptr, (int64_t)0,
+                       h->ab_b0, h->ab_len, partial);
+  }
+  hipLaunchKernelGGL(k_rowinfo_light<RowT>,
+                     dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((MA + 256) / 256, (int64_t)h->num_cu * 8))),
+                     dim3(256), 0, h->stream, MA, (const RowT *)A->row_ptr, (const unsigned long long *)partial, ge,
+                     h->a_rpt64, h->d_info);
+  NSP_HIP(h, hipGetLastError());
+  h->ext_key[0] = A->col;
+  h->ext_key[1] = A->row_ptr;
+  h->ext_key[2] = B->row_ptr;
+  h->ext_key[3] = nullptr;
   h->ext_rows = MA;
   h->ext_nnz = A->nnz;
   h->ext_nprod = -1;
@@ -2183,7 +2256,8 @@ static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, v
   const int64_t dims[4] = {MA, A->nnz, B->nnz, B->cols};
   const bool cached = h->fc_bin >= 0 && std::equal(key, key + 4, h->fc_key) && std::equal(dims, dims + 4, h->fc_dims);
   h->fc_bin = -1;
-  NSP_TRY(run_rowinfo<RowT>(h, A, B));
+  if (cached) NSP_TRY(run_rowinfo_light<RowT>(h, A, B));
+  else NSP_TRY(run_rowinfo<RowT>(h, A, B));
   const Info &hi = *h->h_info;
   int bin = -1;
   if (cached) {
@@ -2214,7 +2288,7 @@ static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, v
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t cbytes = al((size_t)MA * NS * 4), need = cbytes + al((size_t)MA * NS * sizeof(V));
   if (bin < 0 || need > kSpecMaxBytes) {
-    if (cached) NSP_TRY(read_info(h));  // the two-pass path starts from the row info
+    if (cached) return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz);  // the light row info has no bins
     return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz, Batch{}, true);
   }
   if (need > h->spec.bytes) {
@@ -2223,7 +2297,7 @@ static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, v
     h->spec.bytes = 0;
     if (hipMalloc(&h->spec.base, need) != hipSuccess) {
       (void)hipGetLastError();
-      if (cached) NSP_TRY(read_info(h));
+      if (cached) return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz);
       return symbolic_impl<RowT>(h, A, B, c_row_ptr, c_nnz, Batch{}, true);
     }
     h->spec.bytes = need;
@@ -2251,7 +2325,7 @@ static int fused_count_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, v
   if (hi.error) return set_error(h, (int)hi.error, "C has 2^31 or more entries; use NSP_IDX64");
   h->stats.nprod = hi.nprod;
   h->stats.nnz = hi.nnz;
-  h->stats.max_intprod = hi.max_intprod;
+  h->stats.max_intprod = cached ? -1 : hi.max_intprod;  // no per-row counts on the light row info
   h->stats.max_row_nnz = hi.max_nz;
   h->stats.n_sym_bins = kBins;
   h->stats.n_num_bins = kBins;

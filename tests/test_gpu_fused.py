@@ -176,10 +176,13 @@ def test_cached_slot_size_with_a_changed_pattern(nsp, orc):
     h = nsp.Handle()
     dA, dB = nsp.DeviceCSR.from_host(A1), nsp.DeviceCSR.from_host(B)
     C = h.spgemm_into(dA, dB)
-    assert h.stats()["fused"] == 1
+    st1 = h.stats()
+    assert st1["fused"] == 1
     check_against_oracle(A1, B, C, "f64", orc)
-    C = h.spgemm_into(dA, dB)  # same pattern: cached slot size
-    assert h.stats()["fused"] == 1 and h.stats()["host_syncs"] == 1
+    C = h.spgemm_into(dA, dB)  # same pattern: cached slot size, light row info
+    st2 = h.stats()
+    assert st2["fused"] == 1 and st2["host_syncs"] == 1
+    assert st2["nprod"] == st1["nprod"] == int(orc.intprod(A1, B)[1]) and st2["max_intprod"] == -1
     check_against_oracle(A1, B, C, "f64", orc)
     d2 = nsp.DeviceCSR.from_host(A2)
     dA.row_ptr.copy_(d2.row_ptr)
