@@ -1995,8 +1995,12 @@ static int check_csr_args(Handle *h, const char *name, const nsp_csr_t *X) {
   return NSP_OK;
 }
 
+// The counters the host reads back: Info up to the work-queue heads, which
+// only the device uses.
+constexpr size_t kInfoReadBytes = offsetof(Info, queue_num);
+
 static int read_info(Handle *h) {
-  NSP_HIP(h, hipMemcpyAsync(h->h_info, h->d_info, sizeof(Info), hipMemcpyDeviceToHost, h->stream));
+  NSP_HIP(h, hipMemcpyAsync(h->h_info, h->d_info, kInfoReadBytes, hipMemcpyDeviceToHost, h->stream));
   NSP_HIP(h, hipStreamSynchronize(h->stream));
   h->stats.host_syncs++;
   return NSP_OK;
@@ -2557,7 +2561,7 @@ int graph_launch(GraphPlan *gp) {
 
 int graph_result(GraphPlan *gp, int64_t *nnz) {
   Handle *g = &gp->hh;
-  NSP_HIP(g, hipMemcpyAsync(g->h_info, g->d_info, sizeof(Info), hipMemcpyDeviceToHost, gp->launch_stream));
+  NSP_HIP(g, hipMemcpyAsync(g->h_info, g->d_info, kInfoReadBytes, hipMemcpyDeviceToHost, gp->launch_stream));
   NSP_HIP(g, hipStreamSynchronize(gp->launch_stream));
   const Info &hi = *g->h_info;
   if (nnz) *nnz = hi.nnz;
@@ -2657,7 +2661,7 @@ static int noalloc_impl(Handle *h, const nsp_csr_t *A, const nsp_csr_t *B, nsp_c
   h->have_plan = false;  // the handle's own plan is untouched but no longer current
   if (!nnz) return NSP_OK;
   Info hi;
-  NSP_HIP(h, hipMemcpyAsync(&hi, g.d_info, sizeof(Info), hipMemcpyDeviceToHost, h->stream));
+  NSP_HIP(h, hipMemcpyAsync(&hi, g.d_info, kInfoReadBytes, hipMemcpyDeviceToHost, h->stream));
   NSP_HIP(h, hipStreamSynchronize(h->stream));
   *nnz = hi.nnz;
   if (hi.error == NSP_ERR_OVERFLOW && sizeof(RowT) == 4 && hi.nnz > (int64_t)INT32_MAX)
