@@ -1034,6 +1034,12 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
                                            &L.ent[slot]);
 }
 
+constexpr int kNumR = 4;  // B loads in flight per lane, numeric (at most)
+// Loads per lane of a numeric group kernel: a window of GS*R products is
+// held near the bin's row-length bound NS (a row has at least as many
+// products as entries); at least 2.
+constexpr int num_r(int GS, int NS) { return NS / GS >= kNumR ? kNumR : 2; }
+
 // Narrow numeric rows of the wave-level bins 0..5 in one persistent launch
 // driven by row-bin work queues.  The tiles of each bin (RPB = 256 / GS rows
 // of its row list) are split into kQueues equal ranges, one per XCD; queue
@@ -1048,11 +1054,6 @@ __global__ __launch_bounds__(256) void k_num_narrow(int bin_lo, int bin_hi, int6
 // ticket hands it to the block's waves (and frees the pool for the next
 // bin's layout).  Queue heads live in Info (zeroed by k_init), so the launch
 // is valid inside a captured graph.
-constexpr int kNumR = 4;  // B loads in flight per lane, numeric (at most)
-// Loads per lane of a numeric group kernel: a window of GS*R products is
-// held near the bin's row-length bound NS (a row has at least as many
-// products as entries); at least 2.
-constexpr int num_r(int GS, int NS) { return NS / GS >= kNumR ? kNumR : 2; }
 constexpr int kQueues = 8;
 constexpr int kQueueBins = 6;
 constexpr int queue_gs(int b) { return b <= 2 ? 16 : b <= 4 ? 32 : 64; }
@@ -1088,18 +1089,15 @@ __device__ __forceinline__ int queue_ticket(const int64_t *cnt, int q, int64_t k
   return bin;
 }
 
+// FP32 values: at most 128 VGPRs (4 waves per SIMD, as the per-bin kernels
+// run; 2 VGPRs spill); FP64 values take what they need (151, 3 waves).
 template <typename V, bool O32, typename AccN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(V) == 4 ? 4 : 1))) void k_num_narrow_queue(int chunk, Info *__restrict__ info,
-                                                          const int32_t *__restrict__ perm,
-                                                          const int64_t *__restrict__ arpt,
-                                                          const int64_t *__restrict__ ab_b0,
-                                                          const V *__restrict__ aval,
-                                                          const int32_t *__restrict__ ab_len,
-                                                          const int32_t *__restrict__ bcol,
-                                                          const V *__restrict__ bval,
-                                                          const int64_t *__restrict__ crpt,
-                                                          int32_t *__restrict__ ccol,
-                                                          V *__restrict__ cval) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(V) == 4 ? 4 : 1)))
+void k_num_narrow_queue(int chunk, Info *__restrict__ info, const int32_t *__restrict__ perm,
+                        const int64_t *__restrict__ arpt, const int64_t *__restrict__ ab_b0,
+                        const V *__restrict__ aval, const int32_t *__restrict__ ab_len,
+                        const int32_t *__restrict__ bcol, const V *__restrict__ bval,
+                        const int64_t *__restrict__ crpt, int32_t *__restrict__ ccol, V *__restrict__ cval) {
   __shared__ __align__(16) unsigned char pool[queue_pool_bytes<V, AccN>()];
   __shared__ uint32_t s_tk[2];
   __shared__ int64_t cnt[kQueueBins], off[kQueueBins];  // rows of each bin and their place in perm
@@ -1115,9 +1113,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(V) =
   int q = (int)(blockIdx.x % kQueues), tried = 1, it = 0;
   if (threadIdx.x == 0) s_tk[0] = atomicAdd(&heads[q * kQueueStride], 1u);
   __syncthreads();
-  // the ticket is the same in every lane: readfirstlane keeps it (and the
-  // bin switch below) scalar, so that the bins' code paths stay separate
-  // branches instead of being merged into one register budget
+  // the ticket is the same in every lane: readfirstlane keeps it, and the
+  // bin switch below, scalar
   uint32_t tk = __builtin_amdgcn_readfirstlane(s_tk[0]);
   for (;;) {
     int64_t t0 = 0, t1 = 0;
@@ -1134,6 +1131,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(V) =
     }
     uint32_t nxt = 0;
     if (threadIdx.x == 0) nxt = atomicAdd(&heads[q * kQueueStride], 1u);
+    // each bin's path works from an opaque copy of the thread index (an
+    // empty asm): otherwise the compiler hoists every path's loop invariants
+    // (slot, lane, LDS addresses) out of the ticket loop, all six stay live
+    // together, and the kernel needs about 200 VGPRs
 #define QUEUE_BIN(B)                                                                                            \
   case B: {                                                                                                     \
     constexpr int GS = queue_gs(B), NS = queue_ns(B);                                                           \
