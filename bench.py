@@ -212,7 +212,7 @@ def fwd_bwd_leg(dtype: str, m: int = 64, steps: int = 50, warmup: int = 5):
             "ratio": round(t_fb / t_fwd, 3), "fwd_with_symbolic_ms": round(t_full, 4),
             "ratio_vs_fwd_with_symbolic": round(t_fb / t_full, 3),
             "note": "forward on the cached pattern = numeric phase; backward = dC, A^T value "
-                    "gather, dA and dB (2 host syncs)"}
+                    "gather, dA and dB (no host synchronisation: patterns and longest C row cached)"}
 
 
 def batched_leg(count: int = 4096, steps: int = 5, warmup: int = 1, loops: int = 2, params=None,
